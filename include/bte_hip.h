@@ -1,0 +1,109 @@
+/*
+ * C ABI of the MI355X (gfx950) BLS12-381 batch-decryption hot path.
+ *
+ * Every entry point replaces one crate-private L1 function of the reference's weighted_btx /
+ * weighted_pfe crates (cited per declaration).  Conventions:
+ *   - Memory layouts are blst's: Fp = 6 x u64 little-endian limbs in Montgomery form
+ *     (R = 2^384); blst_fp2 {fp[2]}, blst_fp6 {fp2[3]}, blst_fp12 {fp6[2]};
+ *     blst_p1_affine {x, y} (identity = all zero), blst_p1 {x, y, z} (Jacobian, z = 0 is the
+ *     identity), and the same over Fp2 for G2.  Scalars are canonical 32-byte little-endian.
+ *   - Unless a function says otherwise, pointers are DEVICE pointers and work is queued on
+ *     `stream` (a hipStream_t, NULL = the default stream); inputs must stay valid until the
+ *     stream reaches the call's work.
+ *   - Functions return BTE_OK or a nonzero status; they never abort.  Where the reference
+ *     panics on a length mismatch, the C ABI returns BTE_EINVAL.  bte_last_error() gives the
+ *     message of the most recent failure on the calling thread.
+ *   - Entry points are thread-safe across streams.
+ */
+#ifndef BTE_HIP_H
+#define BTE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { BTE_OK = 0, BTE_EINVAL = 1, BTE_ENOMEM = 2, BTE_EDEVICE = 3 };
+
+typedef struct { uint64_t l[6]; } bte_fp;
+typedef struct { bte_fp c[2]; } bte_fp2;
+typedef struct { bte_fp2 c[3]; } bte_fp6;
+typedef struct { bte_fp6 c[2]; } bte_fp12;
+typedef struct { bte_fp x, y; } bte_p1_affine;
+typedef struct { bte_fp x, y, z; } bte_p1;
+typedef struct { bte_fp2 x, y; } bte_p2_affine;
+typedef struct { bte_fp2 x, y, z; } bte_p2;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* bte_last_error(void);
+
+/* Library version string. */
+const char* bte_version(void);
+
+/* ---------------------------------------------------------------------------------------- *
+ * a1/a2: multi-scalar multiplication, sum_i s_i * P_i over the low 255 bits of each scalar
+ * (blst_p{1,2}s_mult_pippenger with nbits = 255).  n == 0 gives the identity.
+ * `out` is one Jacobian point in device memory.  These calls synchronise `stream` (the last
+ * window combination runs on the host) and are therefore not capturable in a graph.
+ * ---------------------------------------------------------------------------------------- */
+
+/* weighted_btx/src/blst_utils.rs:43-67 g1_multi_exp_affine_bytes */
+int bte_g1_msm(const bte_p1_affine* points, const uint8_t* scalars, size_t n, bte_p1* out,
+               void* stream);
+
+/* weighted_btx/src/blst_utils.rs:88-99 g2_multi_exp_affine_bytes */
+int bte_g2_msm(const bte_p2_affine* points, const uint8_t* scalars, size_t n, bte_p2* out,
+               void* stream);
+
+/* weighted_btx/src/blst_utils.rs:106-125 g2_multi_exp_affine_indexed: term i uses
+ * points[indices[i]] (indices bounds-checked against n_points on the device: an index out of
+ * range makes the call return BTE_EINVAL). */
+int bte_g2_msm_indexed(const bte_p2_affine* points, size_t n_points, const uint32_t* indices,
+                       const uint8_t* scalars, size_t n, bte_p2* out, void* stream);
+
+/* Window width override for the Pippenger kernels (0 = the built-in table by size). */
+int bte_msm_set_window(int g1_or_g2, size_t n, int window_bits);
+
+/* ---------------------------------------------------------------------------------------- *
+ * a3: Jacobian -> affine, one shared inversion per chunk (blst_p{1,2}s_to_affine).
+ * ---------------------------------------------------------------------------------------- */
+
+/* weighted_btx/src/blst_utils.rs:11-24 batch_normalize_g1 */
+int bte_g1_batch_normalize(const bte_p1* in, size_t n, bte_p1_affine* out, void* stream);
+/* weighted_btx/src/blst_utils.rs:26-36 batch_normalize_g2 */
+int bte_g2_batch_normalize(const bte_p2* in, size_t n, bte_p2_affine* out, void* stream);
+
+/* ---------------------------------------------------------------------------------------- *
+ * Fixed-base generation of test and key material: out[i] = k_i * G for the standard
+ * generators (affine).  Used to build bases k_i * G for MSM inputs and CRS powers.
+ * ---------------------------------------------------------------------------------------- */
+int bte_g1_generator_mul(const uint8_t* scalars, size_t n, bte_p1_affine* out, void* stream);
+int bte_g2_generator_mul(const uint8_t* scalars, size_t n, bte_p2_affine* out, void* stream);
+
+/* ---------------------------------------------------------------------------------------- *
+ * Measurement: per-phase HIP-event timing of the drivers (off by default) and a throughput
+ * probe of the v_mad_u64_u32 + v_addc_co_u32 pair every field multiplication is built from.
+ * No counterpart in the reference (measurement support).
+ * ---------------------------------------------------------------------------------------- */
+int bte_profile_enable(int on);
+/* "phase=ms;phase=ms..." of the most recent profiled call. */
+int bte_profile_read(char* buf, size_t len);
+/* Measured 32x32->64 multiply-accumulates per second of the whole device. */
+int bte_probe_mac_rate(double* macs_per_second, void* stream);
+
+/* ---------------------------------------------------------------------------------------- *
+ * Device memory helpers for hosts without a HIP binding (the Rust FFI crate).
+ * ---------------------------------------------------------------------------------------- */
+int bte_device_alloc(void** ptr, size_t bytes);
+int bte_device_free(void* ptr);
+int bte_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int bte_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int bte_stream_synchronize(void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BTE_HIP_H */

@@ -1,0 +1,176 @@
+"""GPU parity for the Pippenger MSM, batch normalisation and fixed-base generator multiples
+against the oracle (weighted_btx/src/blst_utils.rs semantics).  Bit-exact on affine points."""
+import random
+
+import pytest
+import torch
+
+from helpers import O, g1_jac_affine, g1_mul_gen, g1_points_bytes, g2_jac_affine, g2_mul_gen, \
+    g2_points_bytes, pack_scalars
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api(dev):
+    from weighted_bte_amd import api as A
+    return A
+
+
+def _rand_bases(rnd, n, mul):
+    ks = [rnd.randrange(1, O.R) for _ in range(n)]
+    return ks, [mul(k) for k in ks]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 64, 200])
+def test_g1_msm_small_vs_oracle(api, n):
+    rnd = random.Random(100 + n)
+    ks, pts = _rand_bases(rnd, n, g1_mul_gen)
+    sc = [rnd.randrange(O.R) for _ in range(n)]
+    out = api.g1_multi_exp_affine_bytes(api.to_device(g1_points_bytes(pts)),
+                                        api.to_device(pack_scalars(sc)))
+    got = g1_jac_affine(api.to_host(out))
+    want = O.G1.to_affine(O.msm(O.G1, pts, sc))
+    assert got == want
+    assert O.g1_compress(got) == O.g1_compress(g1_mul_gen(sum(k * s for k, s in zip(ks, sc))))
+
+
+def test_g1_msm_reference_shape_with_identities(api):
+    """blst_utils.rs:203-249: bases (3i+1)G, identity when i = 0 mod 17, scalars 5i+2."""
+    n = 100
+    ks = [0 if i % 17 == 0 else 3 * i + 1 for i in range(n)]
+    pts = [None if k == 0 else g1_mul_gen(k) for k in ks]
+    sc = [5 * i + 2 for i in range(n)]
+    out = api.g1_multi_exp_affine_bytes(api.to_device(g1_points_bytes(pts)),
+                                        api.to_device(pack_scalars(sc)))
+    assert g1_jac_affine(api.to_host(out)) == g1_mul_gen(sum(k * s for k, s in zip(ks, sc)))
+
+
+def test_g1_msm_edge_cases(api):
+    rnd = random.Random(7)
+    # empty -> identity
+    out = api.g1_multi_exp_affine_bytes(api.to_device(b""), api.to_device(b""))
+    assert g1_jac_affine(api.to_host(out)) is None
+    # duplicate points (forces the doubling branch), zero and r-1 scalars, bit 255 ignored,
+    # a point and its negation cancelling
+    p = g1_mul_gen(12345)
+    pn = (p[0], (-p[1]) % O.P)
+    pts = [p, p, p, pn, p, g1_mul_gen(7), g1_mul_gen(9)]
+    sc = [5, 5, 0, 3, 3, O.R - 1, (1 << 255) | 11]
+    out = api.g1_multi_exp_affine_bytes(api.to_device(g1_points_bytes(pts)),
+                                        api.to_device(pack_scalars(sc)))
+    want = g1_mul_gen(12345 * (5 + 5 + 0 - 3 + 3) + 7 * (O.R - 1) + 9 * 11)
+    assert g1_jac_affine(api.to_host(out)) == want
+    # all scalars identical (every entry in one bucket per window) and sparse scalars
+    n = 300
+    ks, pts = _rand_bases(rnd, n, g1_mul_gen)
+    for sc in ([0xDEADBEEF] * n, [1 << rnd.randrange(255) for _ in range(n)]):
+        out = api.g1_multi_exp_affine_bytes(api.to_device(g1_points_bytes(pts)),
+                                            api.to_device(pack_scalars(sc)))
+        assert g1_jac_affine(api.to_host(out)) == g1_mul_gen(sum(k * s for k, s in zip(ks, sc)))
+
+
+def test_g1_msm_length_mismatch_raises(api):
+    with pytest.raises(ValueError):
+        api.g1_multi_exp_affine_bytes(api.to_device(bytes(96)), api.to_device(bytes(31)))
+
+
+@pytest.mark.parametrize("n", [1, 5, 40])
+def test_g2_msm_small_vs_oracle(api, n):
+    rnd = random.Random(200 + n)
+    ks, pts = _rand_bases(rnd, n, g2_mul_gen)
+    sc = [rnd.randrange(O.R) for _ in range(n)]
+    out = api.g2_multi_exp_affine_bytes(api.to_device(g2_points_bytes(pts)),
+                                        api.to_device(pack_scalars(sc)))
+    got = g2_jac_affine(api.to_host(out))
+    assert got == O.G2.to_affine(O.msm(O.G2, pts, sc))
+    assert O.g2_compress(got) == O.g2_compress(g2_mul_gen(sum(k * s for k, s in zip(ks, sc))))
+
+
+def test_g2_msm_indexed(api):
+    rnd = random.Random(300)
+    npts = 50
+    ks, pts = _rand_bases(rnd, npts, g2_mul_gen)
+    idx = [rnd.randrange(npts) for _ in range(30)] + [3, 3, 3]
+    sc = [rnd.randrange(O.R) for _ in idx]
+    it = torch.tensor(idx, dtype=torch.int32, device="cuda")
+    out = api.g2_multi_exp_affine_indexed(api.to_device(g2_points_bytes(pts)), it,
+                                          api.to_device(pack_scalars(sc)))
+    want = O.msm_indexed(O.G2, pts, idx, sc)
+    assert g2_jac_affine(api.to_host(out)) == O.G2.to_affine(want)
+    from weighted_bte_amd import BteError
+    bad = torch.tensor([0, npts], dtype=torch.int32, device="cuda")
+    with pytest.raises(BteError):
+        api.g2_multi_exp_affine_indexed(api.to_device(g2_points_bytes(pts)), bad,
+                                        api.to_device(pack_scalars([1, 2])))
+
+
+def test_generator_mul_and_batch_normalize(api):
+    rnd = random.Random(400)
+    ks = [0, 1, 2, O.R - 1] + [rnd.randrange(O.R) for _ in range(60)]
+    a1 = api.to_host(api.g1_generator_mul(api.to_device(pack_scalars(ks))))
+    for i, k in enumerate(ks):
+        assert O.g1_affine_from_bytes(a1[96 * i: 96 * i + 96]) == (g1_mul_gen(k) if k % O.R else None)
+    ks2 = ks[:12]
+    a2 = api.to_host(api.g2_generator_mul(api.to_device(pack_scalars(ks2))))
+    for i, k in enumerate(ks2):
+        assert O.g2_affine_from_bytes(a2[192 * i: 192 * i + 192]) == (g2_mul_gen(k) if k % O.R else None)
+    # batch normalisation of non-normalised Jacobian inputs (random Z) and identities
+    jacs = []
+    for k in ks[:40]:
+        p = O.G1.mul(O.g1_gen_jac(), k)
+        z = rnd.randrange(1, O.P)
+        if not O.G1.is_identity(p):
+            x, y, zz = p
+            p = (x * z * z % O.P, y * z * z * z % O.P, zz * z % O.P)
+        jacs.append(p)
+    raw = b"".join(O.g1_jac_to_bytes(p) for p in jacs)
+    aff = api.to_host(api.batch_normalize_g1(api.to_device(raw)))
+    for i, p in enumerate(jacs):
+        assert O.g1_affine_from_bytes(aff[96 * i: 96 * i + 96]) == O.G1.to_affine(p)
+
+
+@pytest.mark.parametrize("logn", [12, 16])
+def test_g1_msm_closed_form_config_sizes(api, logn):
+    """BASELINE config-2 shape: bases k_i*G from the seed, 90% uniform + 10% sparse scalars;
+    MSM == (sum k_i s_i mod r) * G, compared as compressed bytes."""
+    n = 1 << logn
+    rng = O.SeededRng(2, logn)
+    ks = [rng.scalar() for _ in range(n)]
+    sc = O.msm_config_scalars(rng, n)
+    bases = api.g1_generator_mul(api.to_device(pack_scalars(ks)))
+    # spot-check the generated bases against the oracle
+    hb = api.to_host(bases[: 96 * 8])
+    for i in range(8):
+        assert O.g1_affine_from_bytes(hb[96 * i: 96 * i + 96]) == g1_mul_gen(ks[i])
+    out = api.g1_multi_exp_affine_bytes(bases, api.to_device(pack_scalars(sc)))
+    want = g1_mul_gen(sum(k * s for k, s in zip(ks, sc)))
+    assert O.g1_compress(g1_jac_affine(api.to_host(out))) == O.g1_compress(want)
+
+
+def test_g2_msm_closed_form_2_12(api):
+    n = 1 << 12
+    rng = O.SeededRng(3, 12)
+    ks = [rng.scalar() for _ in range(n)]
+    sc = O.msm_config_scalars(rng, n)
+    bases = api.g2_generator_mul(api.to_device(pack_scalars(ks)))
+    out = api.g2_multi_exp_affine_bytes(bases, api.to_device(pack_scalars(sc)))
+    want = g2_mul_gen(sum(k * s for k, s in zip(ks, sc)))
+    assert O.g2_compress(g2_jac_affine(api.to_host(out))) == O.g2_compress(want)
+
+
+@pytest.mark.parametrize("c", [5, 9, 13])
+def test_g1_msm_window_override_same_result(api, c):
+    rnd = random.Random(500 + c)
+    n = 1000
+    ks, pts = _rand_bases(rnd, 40, g1_mul_gen)
+    pts = [pts[i % 40] for i in range(n)]
+    ks = [ks[i % 40] for i in range(n)]
+    sc = [rnd.randrange(O.R) for _ in range(n)]
+    api.set_msm_window(1, n, c)
+    try:
+        out = api.g1_multi_exp_affine_bytes(api.to_device(g1_points_bytes(pts)),
+                                            api.to_device(pack_scalars(sc)))
+    finally:
+        api.set_msm_window(1, n, 0)
+    assert g1_jac_affine(api.to_host(out)) == g1_mul_gen(sum(k * s for k, s in zip(ks, sc)))

@@ -1,0 +1,100 @@
+"""weighted_bte_amd: MI355X (gfx950) BLS12-381 hot path of weighted_btx / weighted_pfe batch
+decryption behind a C ABI (include/bte_hip.h).
+
+This package is a thin ctypes binding over lib/libbte_hip.so plus a Python mirror of the
+reference's crate-private L1 functions (weighted_bte_amd.api).  There is no CPU fallback: if the
+library is not built, loading fails with an error that says how to build it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libbte_hip.so")
+
+_lib = None
+
+
+class BteError(RuntimeError):
+    pass
+
+
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_i = ctypes.c_int
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+_SIGNATURES = {
+    "bte_last_error": [],
+    "bte_version": [],
+    "bte_g1_msm": [_vp, _vp, _sz, _vp, _vp],
+    "bte_g2_msm": [_vp, _vp, _sz, _vp, _vp],
+    "bte_g2_msm_indexed": [_vp, _sz, _vp, _vp, _sz, _vp, _vp],
+    "bte_msm_set_window": [_i, _sz, _i],
+    "bte_g1_batch_normalize": [_vp, _sz, _vp, _vp],
+    "bte_g2_batch_normalize": [_vp, _sz, _vp, _vp],
+    "bte_g1_generator_mul": [_vp, _sz, _vp, _vp],
+    "bte_g2_generator_mul": [_vp, _sz, _vp, _vp],
+    "bte_profile_enable": [_i],
+    "bte_profile_read": [ctypes.c_char_p, _sz],
+    "bte_probe_mac_rate": [ctypes.POINTER(ctypes.c_double), _vp],
+    "bte_device_alloc": [ctypes.POINTER(_vp), _sz],
+    "bte_device_free": [_vp],
+    "bte_memcpy_h2d": [_vp, _vp, _sz, _vp],
+    "bte_memcpy_d2h": [_vp, _vp, _sz, _vp],
+    "bte_stream_synchronize": [_vp],
+}
+_RESTYPES = {"bte_last_error": ctypes.c_char_p, "bte_version": ctypes.c_char_p}
+
+
+def lib():
+    """Load (once) and return the ctypes handle of libbte_hip.so."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BteError(
+                "%s is not built: run `python -m weighted_bte_amd.build` (hipcc, gfx950)" % LIB_PATH)
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, argtypes in _SIGNATURES.items():
+            fn = getattr(handle, name, None)
+            if fn is None:
+                continue
+            fn.argtypes = argtypes
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = handle
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().bte_last_error().decode(errors="replace")
+        raise BteError("bte status %d: %s" % (rc, msg))
+    return rc
+
+
+def call(name, *args):
+    """Call a C ABI entry point and raise BteError on a nonzero status."""
+    return check(getattr(lib(), name)(*args))
+
+
+def profile_enable(on=True):
+    call("bte_profile_enable", 1 if on else 0)
+
+
+def profile_read():
+    """Per-phase milliseconds of the most recent profiled call, as a dict."""
+    buf = ctypes.create_string_buffer(4096)
+    call("bte_profile_read", buf, len(buf))
+    out = {}
+    for item in buf.value.decode().split(";"):
+        if "=" in item:
+            k, v = item.split("=")
+            out[k] = float(v)
+    return out
+
+
+def probe_mac_rate(stream=None):
+    v = ctypes.c_double(0.0)
+    call("bte_probe_mac_rate", ctypes.byref(v), ctypes.c_void_p(stream or 0))
+    return v.value

@@ -1,0 +1,44 @@
+// Shared runtime helpers for the HIP library: status codes, error propagation, launch sizing.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/bte_hip.h"
+
+namespace bte {
+
+// Record the first failure for bte_last_error() and return its status code.
+int set_error(int code, const char* fmt, ...);
+
+#define BTE_HIP(call)                                                                \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return ::bte::set_error(BTE_EDEVICE, "%s:%d %s: %s", __FILE__, __LINE__, #call, \
+                              hipGetErrorString(e_));                                \
+  } while (0)
+
+#define BTE_CHECK_LAUNCH() BTE_HIP(hipGetLastError())
+
+// Optional per-phase timing with HIP events on the stream the kernels run on.  When enabled
+// (bte_profile_enable), drivers mark phase boundaries; after their stream sync the elapsed times
+// are published and bte_profile_read returns them as "name=ms;...".
+struct PhaseTimer {
+  hipStream_t stream;
+  bool on;
+  int n = 0;
+  const char* names[32];
+  hipEvent_t ev[33];
+  explicit PhaseTimer(hipStream_t s);
+  ~PhaseTimer();
+  void begin();
+  void mark(const char* name);  // closes the phase that started at the previous mark
+  void publish();               // call after the stream is synchronised
+};
+
+inline unsigned blocks_for(size_t n, unsigned threads) {
+  return (unsigned)((n + threads - 1) / threads);
+}
+
+}  // namespace bte

@@ -1,0 +1,154 @@
+// Short-Weierstrass group law (a = 0) for G1 over Fp (y^2 = x^3 + 4) and G2 over Fp2 on the
+// M-type twist (y^2 = x^3 + 4(u+1)), Jacobian coordinates x = X/Z^2, y = Y/Z^3, Z = 0 the
+// identity.  Affine points with x = y = 0 are the identity, as in blst_p{1,2}_affine.
+// Struct layouts equal blst_p1 / blst_p1_affine / blst_p2 / blst_p2_affine.
+#pragma once
+#include "tower.hpp"
+
+namespace bte {
+
+// Field dispatch so the same formulas serve Fp and Fp2.
+BTE_DEV fp f_add(const fp& a, const fp& b) { return fp_add(a, b); }
+BTE_DEV fp f_sub(const fp& a, const fp& b) { return fp_sub(a, b); }
+BTE_DEV fp f_mul(const fp& a, const fp& b) { return fp_mul(a, b); }
+BTE_DEV fp f_sqr(const fp& a) { return fp_sqr(a); }
+BTE_DEV fp f_dbl(const fp& a) { return fp_dbl(a); }
+BTE_DEV fp f_neg(const fp& a) { return fp_neg(a); }
+BTE_DEV fp f_inv(const fp& a) { return fp_inv(a); }
+BTE_DEV bool f_is_zero(const fp& a) { return fp_is_zero(a); }
+BTE_DEV bool f_eq(const fp& a, const fp& b) { return fp_eq(a, b); }
+BTE_DEV fp2 f_add(const fp2& a, const fp2& b) { return fp2_add(a, b); }
+BTE_DEV fp2 f_sub(const fp2& a, const fp2& b) { return fp2_sub(a, b); }
+BTE_DEV fp2 f_mul(const fp2& a, const fp2& b) { return fp2_mul(a, b); }
+BTE_DEV fp2 f_sqr(const fp2& a) { return fp2_sqr(a); }
+BTE_DEV fp2 f_dbl(const fp2& a) { return fp2_dbl(a); }
+BTE_DEV fp2 f_neg(const fp2& a) { return fp2_neg(a); }
+BTE_DEV fp2 f_inv(const fp2& a) { return fp2_inv(a); }
+BTE_DEV bool f_is_zero(const fp2& a) { return fp2_is_zero(a); }
+BTE_DEV bool f_eq(const fp2& a, const fp2& b) { return fp2_eq(a, b); }
+
+template <class F> BTE_DEV F f_zero();
+template <> BTE_DEV fp f_zero<fp>() { return fp_zero(); }
+template <> BTE_DEV fp2 f_zero<fp2>() { return fp2_zero(); }
+template <class F> BTE_DEV F f_one();
+template <> BTE_DEV fp f_one<fp>() { return fp_one(); }
+template <> BTE_DEV fp2 f_one<fp2>() { return fp2_one(); }
+
+template <class F> struct jac {
+  F x, y, z;
+};
+template <class F> struct aff {
+  F x, y;
+};
+
+using g1j = jac<fp>;
+using g1a = aff<fp>;
+using g2j = jac<fp2>;
+using g2a = aff<fp2>;
+
+template <class F> BTE_DEV jac<F> jac_identity() { return {f_one<F>(), f_one<F>(), f_zero<F>()}; }
+template <class F> BTE_DEV bool jac_is_identity(const jac<F>& p) { return f_is_zero(p.z); }
+template <class F> BTE_DEV bool aff_is_identity(const aff<F>& p) {
+  return f_is_zero(p.x) && f_is_zero(p.y);
+}
+template <class F> BTE_DEV jac<F> jac_from_aff(const aff<F>& a) {
+  if (aff_is_identity(a)) return jac_identity<F>();
+  return {a.x, a.y, f_one<F>()};
+}
+template <class F> BTE_DEV jac<F> jac_neg(const jac<F>& p) { return {p.x, f_neg(p.y), p.z}; }
+template <class F> BTE_DEV aff<F> aff_neg(const aff<F>& p) {
+  return aff_is_identity(p) ? p : aff<F>{p.x, f_neg(p.y)};
+}
+
+// dbl-2009-l: 2M + 5S.
+template <class F> BTE_DEV jac<F> jac_dbl(const jac<F>& p) {
+  F A = f_sqr(p.x);
+  F B = f_sqr(p.y);
+  F C = f_sqr(B);
+  F D = f_dbl(f_sub(f_sqr(f_add(p.x, B)), f_add(A, C)));
+  F E = f_add(f_dbl(A), A);
+  F Fv = f_sqr(E);
+  jac<F> r;
+  r.x = f_sub(Fv, f_dbl(D));
+  F C8 = f_dbl(f_dbl(f_dbl(C)));
+  r.y = f_sub(f_mul(E, f_sub(D, r.x)), C8);
+  r.z = f_dbl(f_mul(p.y, p.z));
+  return r;  // Z = 0 in, Z = 0 out; Y = 0 cannot occur for points of order r.
+}
+
+// add-2007-bl: 11M + 5S, with the exceptional cases handled.
+template <class F> BTE_DEV jac<F> jac_add(const jac<F>& p, const jac<F>& q) {
+  if (jac_is_identity(p)) return q;
+  if (jac_is_identity(q)) return p;
+  F Z1Z1 = f_sqr(p.z);
+  F Z2Z2 = f_sqr(q.z);
+  F U1 = f_mul(p.x, Z2Z2);
+  F U2 = f_mul(q.x, Z1Z1);
+  F S1 = f_mul(f_mul(p.y, q.z), Z2Z2);
+  F S2 = f_mul(f_mul(q.y, p.z), Z1Z1);
+  F H = f_sub(U2, U1);
+  F R = f_sub(S2, S1);
+  if (f_is_zero(H)) {
+    if (f_is_zero(R)) return jac_dbl(p);
+    return jac_identity<F>();
+  }
+  F I = f_sqr(f_dbl(H));
+  F J = f_mul(H, I);
+  F r2 = f_dbl(R);
+  F V = f_mul(U1, I);
+  jac<F> o;
+  o.x = f_sub(f_sub(f_sqr(r2), J), f_dbl(V));
+  o.y = f_sub(f_mul(r2, f_sub(V, o.x)), f_dbl(f_mul(S1, J)));
+  o.z = f_mul(f_sub(f_sqr(f_add(p.z, q.z)), f_add(Z1Z1, Z2Z2)), H);
+  return o;
+}
+
+// madd-2007-bl: 7M + 4S, q affine (identity allowed).
+template <class F> BTE_DEV jac<F> jac_add_aff(const jac<F>& p, const aff<F>& q) {
+  if (aff_is_identity(q)) return p;
+  if (jac_is_identity(p)) return {q.x, q.y, f_one<F>()};
+  F Z1Z1 = f_sqr(p.z);
+  F U2 = f_mul(q.x, Z1Z1);
+  F S2 = f_mul(f_mul(q.y, p.z), Z1Z1);
+  F H = f_sub(U2, p.x);
+  F R = f_sub(S2, p.y);
+  if (f_is_zero(H)) {
+    if (f_is_zero(R)) return jac_dbl(p);
+    return jac_identity<F>();
+  }
+  F HH = f_sqr(H);
+  F I = f_dbl(f_dbl(HH));
+  F J = f_mul(H, I);
+  F r2 = f_dbl(R);
+  F V = f_mul(p.x, I);
+  jac<F> o;
+  o.x = f_sub(f_sub(f_sqr(r2), J), f_dbl(V));
+  o.y = f_sub(f_mul(r2, f_sub(V, o.x)), f_dbl(f_mul(p.y, J)));
+  o.z = f_sub(f_sub(f_sqr(f_add(p.z, H)), Z1Z1), HH);
+  return o;
+}
+
+template <class F> BTE_DEV jac<F> jac_sub(const jac<F>& p, const jac<F>& q) {
+  return jac_add(p, jac_neg(q));
+}
+
+// Affine form using one inversion (the identity maps to x = y = 0).
+template <class F> BTE_DEV aff<F> jac_to_aff(const jac<F>& p) {
+  if (jac_is_identity(p)) return {f_zero<F>(), f_zero<F>()};
+  F zi = f_inv(p.z);
+  F zi2 = f_sqr(zi);
+  return {f_mul(p.x, zi2), f_mul(p.y, f_mul(zi2, zi))};
+}
+
+// k * p for a scalar given as nwords little-endian u32 words (left-to-right double-and-add).
+template <class F> BTE_DEV jac<F> jac_mul_scalar(const jac<F>& p, const uint32_t* k, int nwords) {
+  jac<F> acc = jac_identity<F>();
+#pragma unroll 1
+  for (int i = nwords * 32 - 1; i >= 0; --i) {
+    acc = jac_dbl(acc);
+    if ((k[i >> 5] >> (i & 31)) & 1u) acc = jac_add(acc, p);
+  }
+  return acc;
+}
+
+}  // namespace bte

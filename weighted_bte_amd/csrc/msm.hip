@@ -1,0 +1,456 @@
+// Pippenger multi-scalar multiplication over G1 and G2 for gfx950.
+//
+// Replaces blst_p{1,2}s_mult_pippenger as called by weighted_btx/src/blst_utils.rs:43-67
+// (g1_multi_exp_affine_bytes), :88-99 (g2_multi_exp_affine_bytes) and :106-152
+// (g2_multi_exp_affine_indexed / _raw): sum_i s_i * P_i over the low 255 bits of each
+// 32-byte little-endian scalar; an empty input gives the identity.
+//
+// Pipeline (one stream, all device work except the last serial combination):
+//   1. digits    one thread per point: signed c-bit window digits d in [-(2^(c-1)-1), 2^(c-1)],
+//                global histogram of |d| per (window, bucket)
+//   2. scan      exclusive scan of the histogram -> bucket offsets in one flat sorted array
+//   3. scatter   counting-sort placement of (point index, sign) into bucket order
+//   4. accumulate  the sorted array is cut into chunks of K entries, one thread per chunk; a
+//                thread sums each run of equal bucket with mixed Jacobian+affine additions.  A
+//                bucket wholly inside a chunk is written directly; pieces of buckets that
+//                cross chunk boundaries go to per-chunk head/tail slots.  Chunking keeps the
+//                work per thread bounded however skewed the buckets are (sparse scalars pile
+//                onto power-of-two buckets).
+//   5. fixup     one thread per bucket that crosses chunks adds its pieces
+//   6. segsum    per window, segments of L buckets: running sums give
+//                acc_j = sum_k (k+1) B_{jL+k} and run_j = sum_k B_{jL+k}
+//   7. window sums  per window one workgroup tree-sums A = sum_j acc_j and, for each bit b of
+//                the segment index, D_b = sum_{j: bit b of j} run_j, so that
+//                T_w = A + L * sum_b 2^b D_b
+//   8. host      sum_w 2^(c w) T_w by one Horner pass over at most 256 doublings; this serial
+//                tail is ~0.1 ms on a host core and would cost milliseconds in one GPU lane.
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "curve.hpp"
+#include "host_field.hpp"
+
+namespace bte {
+
+static constexpr int MSM_CHUNK = 32;  // sorted entries per accumulation thread
+
+template <class F> struct MsmTraits;
+template <> struct MsmTraits<fp> {
+  using HF = bteh::Fp;
+  static constexpr int REDUCE_THREADS = 256;
+};
+template <> struct MsmTraits<fp2> {
+  using HF = bteh::Fp2;
+  static constexpr int REDUCE_THREADS = 128;
+};
+
+struct MsmShape {
+  int c;      // window bits
+  int nw;     // windows, ceil(256 / c)
+  uint32_t M; // buckets per window, 2^(c-1)
+  int logL;   // log2 of buckets per reduction segment
+  int logS;   // log2 of segments per window
+};
+
+static int g_window_override[2][64];  // [curve][log2 n] -> c (0 = default)
+static std::mutex g_window_mu;
+
+static int floor_log2(size_t n) {
+  int lg = 0;
+  while (lg < 63 && ((size_t)1 << (lg + 1)) <= n) ++lg;
+  return lg;
+}
+
+static MsmShape msm_shape(int curve, size_t n) {
+  int lg = floor_log2(n);
+  int c;
+  {
+    std::lock_guard<std::mutex> lk(g_window_mu);
+    c = g_window_override[curve][lg];
+  }
+  if (c <= 0) c = std::max(4, std::min(16, lg - 4));
+  MsmShape s;
+  s.c = c;
+  s.nw = (256 + c - 1) / c;
+  s.M = 1u << (c - 1);
+  s.logL = std::min(c - 1, 3);
+  s.logS = (c - 1) - s.logL;
+  return s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_msm_digits(const uint32_t* __restrict__ scalars, size_t n, int c, int nw,
+                             uint32_t M, int32_t* __restrict__ digits,
+                             uint32_t* __restrict__ counts) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t s[9];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s[k] = scalars[8 * i + k];
+  s[7] &= 0x7fffffffu;  // nbits = 255
+  s[8] = 0;
+  const uint32_t mask = (1u << c) - 1u;
+  int carry = 0;
+  for (int w = 0; w < nw; ++w) {
+    const int bit = w * c;
+    const int word = bit >> 5, off = bit & 31;
+    uint64_t two = (uint64_t)s[word] | ((uint64_t)s[word + 1] << 32);
+    int d = (int)((two >> off) & mask) + carry;
+    if (d > (int)M) {
+      d -= (int)(1u << c);
+      carry = 1;
+    } else {
+      carry = 0;
+    }
+    digits[(size_t)w * n + i] = d;
+    if (d) atomicAdd(&counts[(size_t)w * M + (uint32_t)(d < 0 ? -d : d) - 1u], 1u);
+  }
+}
+
+// Exclusive scan of counts[0..nb) into offsets[0..nb], offsets[nb] = total.  Three passes over
+// 1024-element tiles (256 threads x 4).
+__device__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* sh, uint32_t* total) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    uint32_t t = tid >= off ? sh[tid - off] : 0u;
+    __syncthreads();
+    sh[tid] += t;
+    __syncthreads();
+  }
+  uint32_t incl = sh[tid];
+  *total = sh[255];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void k_scan_tiles(const uint32_t* __restrict__ in, size_t nb, uint32_t* __restrict__ out,
+                             uint32_t* __restrict__ tile_sums) {
+  __shared__ uint32_t sh[256];
+  size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x * 4;
+  uint32_t v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = base + k < nb ? in[base + k] : 0u;
+  uint32_t local = v[0] + v[1] + v[2] + v[3];
+  uint32_t total;
+  uint32_t ex = block_exclusive_scan_256(local, sh, &total);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < nb) out[base + k] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_tile_sums(uint32_t* tile_sums, int ntiles, uint32_t* out_total) {
+  __shared__ uint32_t sh[256];
+  uint32_t carry = 0;
+  for (int base = 0; base < ntiles; base += 1024) {
+    int idx = base + threadIdx.x * 4;
+    uint32_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = idx + k < ntiles ? tile_sums[idx + k] : 0u;
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan_256(v[0] + v[1] + v[2] + v[3], sh, &total);
+    ex += carry;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (idx + k < ntiles) tile_sums[idx + k] = ex;
+      ex += v[k];
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ void k_scan_add(uint32_t* out, size_t nb, const uint32_t* tile_sums) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < nb) out[i] += tile_sums[i / 1024];
+}
+
+__global__ void k_msm_scatter(const int32_t* __restrict__ digits, size_t n, int nw, uint32_t M,
+                              const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
+                              uint32_t* __restrict__ sorted) {
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= n * (size_t)nw) return;
+  const int d = digits[t];
+  if (!d) return;
+  const size_t w = t / n;
+  const size_t i = t - w * n;
+  const uint32_t b = (uint32_t)(w * M) + (uint32_t)(d < 0 ? -d : d) - 1u;
+  const uint32_t pos = offsets[b] + atomicAdd(&cursor[b], 1u);
+  sorted[pos] = (uint32_t)i | (d < 0 ? 0x80000000u : 0u);
+}
+
+template <class F>
+__global__ void __launch_bounds__(64) k_msm_accumulate(const aff<F>* __restrict__ points, const uint32_t* __restrict__ index,
+                                 size_t n_points, const uint32_t* __restrict__ sorted,
+                                 const uint32_t* __restrict__ offsets, uint32_t nb,
+                                 jac<F>* __restrict__ buckets, jac<F>* __restrict__ heads,
+                                 jac<F>* __restrict__ tails, int* __restrict__ err) {
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const uint32_t E = offsets[nb];
+  const size_t start = t * MSM_CHUNK;
+  if (start >= E) return;
+  const size_t stop = start + MSM_CHUNK < E ? start + MSM_CHUNK : E;
+  // Bucket holding `start`: offsets[lo] <= start < offsets[hi], narrowed to hi = lo + 1.
+  uint32_t lo = 0, hi = nb;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (offsets[mid] <= start) lo = mid; else hi = mid;
+  }
+  uint32_t b = lo, bs = offsets[b], be = offsets[b + 1];
+  jac<F> acc = jac_identity<F>();
+  auto emit = [&]() {
+    if (bs < start) heads[t] = acc;
+    else if (be > stop) tails[t] = acc;
+    else buckets[b] = acc;
+  };
+  for (size_t pos = start; pos < stop; ++pos) {
+    if (pos == be) {
+      emit();
+      acc = jac_identity<F>();
+      do {
+        ++b;
+        bs = be;
+        be = offsets[b + 1];
+      } while (be == bs);
+    }
+    const uint32_t e = sorted[pos];
+    uint32_t idx = e & 0x7fffffffu;
+    if (index) {
+      idx = index[idx];
+      if (idx >= n_points) {
+        *err = 1;
+        continue;
+      }
+    }
+    aff<F> q = points[idx];
+    if (e >> 31) q = aff_neg(q);
+    acc = jac_add_aff(acc, q);
+  }
+  emit();
+}
+
+template <class F>
+__global__ void __launch_bounds__(64) k_msm_fixup(const uint32_t* __restrict__ offsets, uint32_t nb,
+                            jac<F>* __restrict__ buckets, const jac<F>* __restrict__ heads,
+                            const jac<F>* __restrict__ tails) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const uint32_t bs = offsets[b], be = offsets[b + 1];
+  if (bs == be) {
+    buckets[b] = jac_identity<F>();
+    return;
+  }
+  const uint32_t cs = bs / MSM_CHUNK, ce = (be - 1) / MSM_CHUNK;
+  if (cs == ce) return;
+  jac<F> acc = tails[cs];
+  for (uint32_t c = cs + 1; c <= ce; ++c) acc = jac_add(acc, heads[c]);
+  buckets[b] = acc;
+}
+
+template <class F>
+__global__ void __launch_bounds__(64) k_msm_segsum(const jac<F>* __restrict__ buckets, int nw, uint32_t M, int logL,
+                             int logS, jac<F>* __restrict__ seg_acc, jac<F>* __restrict__ seg_run) {
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t S = (size_t)1 << logS;
+  if (t >= (size_t)nw * S) return;
+  const size_t w = t >> logS, j = t & (S - 1);
+  const int L = 1 << logL;
+  const jac<F>* base = buckets + w * M + (j << logL);
+  jac<F> run = jac_identity<F>(), acc = jac_identity<F>();
+  for (int k = L - 1; k >= 0; --k) {
+    run = jac_add(run, base[k]);
+    acc = jac_add(acc, run);
+  }
+  seg_acc[t] = acc;
+  seg_run[t] = run;
+}
+
+template <class F, int T>
+__global__ void __launch_bounds__(T) k_msm_window_sums(const jac<F>* __restrict__ seg_acc,
+                                                       const jac<F>* __restrict__ seg_run, int logS,
+                                                       jac<F>* __restrict__ out) {
+  __shared__ jac<F> sh[T];
+  const int nsum = 1 + logS;
+  const int w = blockIdx.x / nsum, sid = blockIdx.x % nsum;
+  const size_t S = (size_t)1 << logS;
+  const int tid = threadIdx.x;
+  jac<F> acc = jac_identity<F>();
+  for (size_t j = tid; j < S; j += T) {
+    if (sid == 0) acc = jac_add(acc, seg_acc[w * S + j]);
+    else if ((j >> (sid - 1)) & 1) acc = jac_add(acc, seg_run[w * S + j]);
+  }
+  sh[tid] = acc;
+  __syncthreads();
+  for (int s = T / 2; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] = jac_add(sh[tid], sh[tid + s]);
+    __syncthreads();
+  }
+  if (tid == 0) out[blockIdx.x] = sh[0];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host driver
+// ---------------------------------------------------------------------------------------------
+template <class T> static int dalloc(T** p, size_t count, hipStream_t s) {
+  if (count == 0) count = 1;
+  hipError_t e = hipMallocAsync((void**)p, count * sizeof(T), s);
+  if (e != hipSuccess) return set_error(BTE_ENOMEM, "hipMallocAsync(%zu): %s", count * sizeof(T),
+                                        hipGetErrorString(e));
+  return BTE_OK;
+}
+
+struct AsyncFreeList {
+  hipStream_t s;
+  std::vector<void*> ptrs;
+  ~AsyncFreeList() {
+    for (void* p : ptrs) (void)hipFreeAsync(p, s);
+  }
+  template <class T> int alloc(T** p, size_t count) {
+    int rc = dalloc(p, count, s);
+    if (rc == BTE_OK) ptrs.push_back((void*)*p);
+    return rc;
+  }
+};
+
+template <class F>
+static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint32_t* index,
+                   const uint8_t* scalars, size_t n, jac<F>* out, hipStream_t s) {
+  using HF = typename MsmTraits<F>::HF;
+  if (n == 0) {
+    bteh::Jac<HF> id = bteh::jac_identity<HF>();
+    BTE_HIP(hipMemcpyAsync(out, &id, sizeof(id), hipMemcpyHostToDevice, s));
+    BTE_HIP(hipStreamSynchronize(s));
+    return BTE_OK;
+  }
+  if (!points || !scalars || !out) return set_error(BTE_EINVAL, "msm: null pointer");
+  if (((uintptr_t)scalars & 3) != 0) return set_error(BTE_EINVAL, "msm: scalars must be 4-byte aligned");
+  if (n > 0x7fffffffu / 2) return set_error(BTE_EINVAL, "msm: n too large");
+  const MsmShape sh = msm_shape(curve, n);
+  const uint32_t nb = (uint32_t)sh.nw * sh.M;
+  const size_t E_max = n * (size_t)sh.nw;
+  const size_t nchunks = (E_max + MSM_CHUNK - 1) / MSM_CHUNK;
+  const size_t S = (size_t)1 << sh.logS;
+  const int nsum = 1 + sh.logS;
+  const int ntiles = (int)((nb + 1023) / 1024);
+  if (ntiles > (1 << 20)) return set_error(BTE_EINVAL, "msm: window too wide");
+
+  AsyncFreeList fl{s, {}};
+  int32_t* digits;
+  uint32_t *counts, *offsets, *cursor, *sorted, *tile_sums;
+  jac<F> *buckets, *heads, *tails, *seg_acc, *seg_run, *wsum;
+  int* err;
+  int rc;
+  if ((rc = fl.alloc(&digits, E_max)) || (rc = fl.alloc(&counts, nb)) ||
+      (rc = fl.alloc(&offsets, (size_t)nb + 1)) || (rc = fl.alloc(&cursor, nb)) ||
+      (rc = fl.alloc(&sorted, E_max)) || (rc = fl.alloc(&tile_sums, ntiles)) ||
+      (rc = fl.alloc(&buckets, nb)) || (rc = fl.alloc(&heads, nchunks)) ||
+      (rc = fl.alloc(&tails, nchunks)) || (rc = fl.alloc(&seg_acc, sh.nw * S)) ||
+      (rc = fl.alloc(&seg_run, sh.nw * S)) || (rc = fl.alloc(&wsum, (size_t)sh.nw * nsum)) ||
+      (rc = fl.alloc(&err, 1)))
+    return rc;
+
+  PhaseTimer pt(s);
+  pt.begin();
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, counts, (size_t)nb, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, cursor, (size_t)nb, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)err, (size_t)1, 0u);
+  hipLaunchKernelGGL(k_msm_digits, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                     (const uint32_t*)scalars, n, sh.c, sh.nw, sh.M, digits, counts);
+  pt.mark("digits");
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(256), 0, s, counts, (size_t)nb, offsets, tile_sums);
+  hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(256), 0, s, tile_sums, ntiles, offsets + nb);
+  hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(nb, 256)), dim3(256), 0, s, offsets, (size_t)nb,
+                     (const uint32_t*)tile_sums);
+  hipLaunchKernelGGL(k_msm_scatter, dim3(blocks_for(E_max, 256)), dim3(256), 0, s,
+                     (const int32_t*)digits, n, sh.nw, sh.M, (const uint32_t*)offsets, cursor, sorted);
+  pt.mark("sort");
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_accumulate<F>), dim3(blocks_for(nchunks, 64)), dim3(64), 0, s,
+                     points, index, n_points, (const uint32_t*)sorted, (const uint32_t*)offsets, nb,
+                     buckets, heads, tails, err);
+  pt.mark("accumulate");
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup<F>), dim3(blocks_for(nb, 64)), dim3(64), 0, s,
+                     (const uint32_t*)offsets, nb, buckets, (const jac<F>*)heads, (const jac<F>*)tails);
+  pt.mark("fixup");
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_segsum<F>), dim3(blocks_for(sh.nw * S, 64)), dim3(64), 0, s,
+                     (const jac<F>*)buckets, sh.nw, sh.M, sh.logL, sh.logS, seg_acc, seg_run);
+  pt.mark("segsum");
+  constexpr int RT = MsmTraits<F>::REDUCE_THREADS;
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_window_sums<F, RT>), dim3(sh.nw * nsum), dim3(RT), 0, s,
+                     (const jac<F>*)seg_acc, (const jac<F>*)seg_run, sh.logS, wsum);
+  pt.mark("window_sums");
+  BTE_CHECK_LAUNCH();
+
+  std::vector<bteh::Jac<HF>> hw((size_t)sh.nw * nsum);
+  int herr = 0;
+  BTE_HIP(hipMemcpyAsync(hw.data(), wsum, hw.size() * sizeof(hw[0]), hipMemcpyDeviceToHost, s));
+  BTE_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  BTE_HIP(hipStreamSynchronize(s));
+  pt.publish();
+  if (herr) return set_error(BTE_EINVAL, "msm: point index out of range");
+
+  // Horner over the exponent: A_w sits at 2^(c w), D_{w,b} at 2^(c w + logL + b).
+  const int top = sh.c * sh.nw;
+  std::vector<std::vector<int>> at(top + 1);
+  for (int w = 0; w < sh.nw; ++w) {
+    at[sh.c * w].push_back(w * nsum);
+    for (int b = 0; b < sh.logS; ++b) at[sh.c * w + sh.logL + b].push_back(w * nsum + 1 + b);
+  }
+  bteh::Jac<HF> acc = bteh::jac_identity<HF>();
+  for (int e = top; e >= 0; --e) {
+    acc = bteh::jac_dbl(acc);
+    for (int k : at[e]) acc = bteh::jac_add(acc, hw[k]);
+  }
+  BTE_HIP(hipMemcpyAsync(out, &acc, sizeof(acc), hipMemcpyHostToDevice, s));
+  BTE_HIP(hipStreamSynchronize(s));
+  return BTE_OK;
+}
+
+static_assert(sizeof(bteh::Jac<bteh::Fp>) == sizeof(bte_p1), "layout");
+static_assert(sizeof(bteh::Jac<bteh::Fp2>) == sizeof(bte_p2), "layout");
+static_assert(sizeof(jac<fp>) == sizeof(bte_p1), "layout");
+static_assert(sizeof(aff<fp2>) == sizeof(bte_p2_affine), "layout");
+
+}  // namespace bte
+
+using namespace bte;
+
+extern "C" int bte_g1_msm(const bte_p1_affine* points, const uint8_t* scalars, size_t n, bte_p1* out,
+                          void* stream) {
+  return msm_run<fp>(0, (const aff<fp>*)points, n, nullptr, scalars, n, (jac<fp>*)out,
+                     (hipStream_t)stream);
+}
+
+extern "C" int bte_g2_msm(const bte_p2_affine* points, const uint8_t* scalars, size_t n, bte_p2* out,
+                          void* stream) {
+  return msm_run<fp2>(1, (const aff<fp2>*)points, n, nullptr, scalars, n, (jac<fp2>*)out,
+                      (hipStream_t)stream);
+}
+
+extern "C" int bte_g2_msm_indexed(const bte_p2_affine* points, size_t n_points, const uint32_t* indices,
+                                  const uint8_t* scalars, size_t n, bte_p2* out, void* stream) {
+  if (n && !indices) return set_error(BTE_EINVAL, "msm_indexed: null indices");
+  return msm_run<fp2>(1, (const aff<fp2>*)points, n_points, indices, scalars, n, (jac<fp2>*)out,
+                      (hipStream_t)stream);
+}
+
+extern "C" int bte_msm_set_window(int g1_or_g2, size_t n, int window_bits) {
+  if ((g1_or_g2 != 1 && g1_or_g2 != 2) || window_bits < 0 || window_bits > 20 ||
+      (window_bits > 0 && window_bits < 2))
+    return set_error(BTE_EINVAL, "msm_set_window: bad arguments");
+  std::lock_guard<std::mutex> lk(g_window_mu);
+  g_window_override[g1_or_g2 - 1][floor_log2(n)] = window_bits;
+  return BTE_OK;
+}
