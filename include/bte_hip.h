@@ -76,6 +76,57 @@ int bte_g1_batch_normalize(const bte_p1* in, size_t n, bte_p1_affine* out, void*
 int bte_g2_batch_normalize(const bte_p2* in, size_t n, bte_p2_affine* out, void* stream);
 
 /* ---------------------------------------------------------------------------------------- *
+ * a4-a9: radix-2 transforms over the Fr domain of size n (a power of two <= 2^32), in place.
+ * Natural-order DFT X_k = sum_j x_j w^(jk), w = ROOT_OF_UNITY^(2^(32 - log n)); `inverse` uses
+ * w^-1; `scaled` multiplies by n^-1 (inverse only).  A size that is not a power of two returns
+ * BTE_EINVAL (the reference's Error::InvalidBatchSize).
+ * ---------------------------------------------------------------------------------------- */
+
+/* weighted_btx/src/fft.rs:197-234 Radix2Domain::{fft, ifft, ifft_unscaled} over G1Projective */
+int bte_fft_g1(bte_p1* values, size_t n, int inverse, int scaled, void* stream);
+/* same over G2Projective */
+int bte_fft_g2(bte_p2* values, size_t n, int inverse, int scaled, void* stream);
+/* same over Gt (fft.rs:197-234 generic path, fft.rs:242-274 ifft_gt_unscaled) */
+int bte_fft_gt(bte_fp12* values, size_t n, int inverse, int scaled, void* stream);
+/* weighted_pfe/src/fft.rs:324-331 fft_cyclotomic / ifft_cyclotomic_unscaled */
+int bte_fft_cyclotomic(bte_fp12* values, size_t n, int inverse, void* stream);
+/* weighted_btx/src/fft.rs:284-350 ifft_cyclotomic_prefix_unscaled: values[0..output_count)
+ * receive the first outputs (output_count <= n/2); the rest of the buffer is scratch. */
+int bte_ifft_cyclotomic_prefix_unscaled(bte_fp12* values, size_t n, size_t output_count, void* stream);
+/* weighted_btx/src/fft.rs:65-71 cyclotomic_mul_scalar_decomposed, element-wise:
+ * values[i] <- decomposed action of scalars[i] (canonical 32-byte LE, HOST memory). */
+int bte_cyclotomic_pow_scalars(bte_fp12* values, const uint8_t* host_scalars, size_t n, void* stream);
+/* weighted_pfe/src/fft.rs:671-697 mul_cyclotomic_signed_small, element-wise (device i64). */
+int bte_cyclotomic_pow_small(bte_fp12* values, const int64_t* coefficients, size_t n, void* stream);
+
+/* ---------------------------------------------------------------------------------------- *
+ * a10-a14: Miller loops and final exponentiation.  Line tables are 68 x bte_fp6 per G2 point
+ * (19,584 bytes, the size of the reference's PreparedG2Lines lines); an identity point gives an
+ * all-zero table.  Raw Miller values are only meaningful after the final exponentiation.
+ * ---------------------------------------------------------------------------------------- */
+
+/* weighted_btx/src/final_exponentiation.rs:186-195 PreparedG2Lines::from_affine */
+int bte_g2_prepare_lines(const bte_p2_affine* q, size_t n, bte_fp6* lines, void* stream);
+/* final_exponentiation.rs:207-219 PreparedG2Lines::miller_loop, pair i = (p[i], lines[68 i..]) */
+int bte_miller_loop_prepared(const bte_fp6* lines, const bte_p1_affine* p, size_t n, bte_fp12* out,
+                             void* stream);
+/* one unprepared Miller loop per pair (p[i], q[i]) */
+int bte_miller_loop_pairs(const bte_p1_affine* p, const bte_p2_affine* q, size_t n, bte_fp12* out,
+                          void* stream);
+/* final_exponentiation.rs:229-273 unprepared_multi_miller_loop, batched over segments of uneven
+ * arity: out[s] = product of the loops of pairs seg_offsets[s] .. seg_offsets[s+1] (device u64
+ * offsets, seg_offsets[nseg] = npairs); pairs with an identity are skipped, empty = one. */
+int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine* q, size_t npairs,
+                          const uint64_t* seg_offsets, size_t nseg, bte_fp12* out, void* stream);
+/* final_exponentiation.rs:276-305 easy_final_exponentiation / batch_easy_final_exponentiation
+ * (inputs nonzero; in and out must not alias) */
+int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
+/* final_exponentiation.rs:308-336 hard_final_exponentiation (in place allowed) */
+int bte_final_exp_hard(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
+/* final_exponentiation.rs:348-354 full_final_exponentiation (in and out must not alias) */
+int bte_final_exp(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
+
+/* ---------------------------------------------------------------------------------------- *
  * Fixed-base generation of test and key material: out[i] = k_i * G for the standard
  * generators (affine).  Used to build bases k_i * G for MSM inputs and CRS powers.
  * ---------------------------------------------------------------------------------------- */

@@ -1001,3 +1001,18 @@ def msm_config_scalars(rng, n, sparse_fraction=0.1):
     n_sparse = int(n * sparse_fraction)
     out = [rng.scalar() for _ in range(n - n_sparse)] + [rng.sparse_scalar() for _ in range(n_sparse)]
     return out
+
+
+def fft_cyclotomic(values, inverse=False):
+    """weighted_pfe/src/fft.rs:324-365: full unscaled transform over the cyclotomic subgroup,
+    twiddles applied by the decomposed action."""
+    d = Radix2Domain(len(values))
+    g = d.generator_inverse if inverse else d.generator
+    return fft_generic(values, g, f12_mul, lambda a, b: f12_mul(a, f12_conj(b)),
+                       cyclotomic_mul_scalar_decomposed)
+
+
+def mul_cyclotomic_signed_small(v, c):
+    """weighted_pfe/src/fft.rs:671-697 (an exact power; negative -> conjugate)."""
+    r = f12_pow(v, abs(c))
+    return f12_conj(r) if c < 0 else r

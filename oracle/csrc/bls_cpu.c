@@ -1,0 +1,313 @@
+/*
+ * CPU oracle, C restatement (TEST INFRASTRUCTURE ONLY: used by tests/ as a checker and by
+ * bench.py's cpu_baseline leg; the product never links it).
+ *
+ * BLS12-381 Fp / Fp2 Montgomery arithmetic on 6 x 64-bit limbs (R = 2^384, the blst layout),
+ * Jacobian G1/G2 group law and a Pippenger MSM parallelised with OpenMP over windows, following
+ * the semantics of blst_p{1,2}s_mult_pippenger as the reference calls it
+ * (weighted_btx/src/blst_utils.rs:43-152): sum_i s_i P_i over the low 255 bits of 32-byte
+ * little-endian scalars; affine inputs with x = y = 0 are the identity.
+ *
+ * Pinned in tests/test_oracle_cpu.py against the Python oracle (oracle/bls12_381.py) on random
+ * inputs and against the closed form sum(k_i s_i) G.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+
+typedef struct { uint64_t l[6]; } fp;
+typedef struct { fp c0, c1; } fp2;
+
+static const fp P = {{0xb9feffffffffaaabull, 0x1eabfffeb153ffffull, 0x6730d2a0f6b0f624ull,
+                      0x64774b84f38512bfull, 0x4b1ba7b6434bacd7ull, 0x1a0111ea397fe69aull}};
+static const uint64_t PINV = 0x89f3fffcfffcfffdull;
+static const fp ONE = {{0x760900000002fffdull, 0xebf4000bc40c0002ull, 0x5f48985753c758baull,
+                        0x77ce585370525745ull, 0x5c071a97a256ec6dull, 0x15f65ec3fa80e493ull}};
+
+static int fp_geq(const fp* a, const fp* b) {
+  for (int i = 5; i >= 0; --i)
+    if (a->l[i] != b->l[i]) return a->l[i] > b->l[i];
+  return 1;
+}
+static void fp_sub_raw(fp* r, const fp* a, const fp* b) {
+  uint64_t br = 0;
+  for (int i = 0; i < 6; ++i) {
+    u128 d = (u128)a->l[i] - b->l[i] - br;
+    r->l[i] = (uint64_t)d;
+    br = (uint64_t)(d >> 64) & 1;
+  }
+}
+static fp fp_add(fp a, fp b) {
+  fp r;
+  uint64_t c = 0;
+  for (int i = 0; i < 6; ++i) {
+    u128 s = (u128)a.l[i] + b.l[i] + c;
+    r.l[i] = (uint64_t)s;
+    c = (uint64_t)(s >> 64);
+  }
+  if (fp_geq(&r, &P)) fp_sub_raw(&r, &r, &P);
+  return r;
+}
+static fp fp_sub(fp a, fp b) {
+  fp r;
+  if (fp_geq(&a, &b)) {
+    fp_sub_raw(&r, &a, &b);
+  } else {
+    fp t;
+    uint64_t c = 0;
+    for (int i = 0; i < 6; ++i) {
+      u128 s = (u128)a.l[i] + P.l[i] + c;
+      t.l[i] = (uint64_t)s;
+      c = (uint64_t)(s >> 64);
+    }
+    fp_sub_raw(&r, &t, &b);
+  }
+  return r;
+}
+static int fp_is_zero(fp a) { return (a.l[0] | a.l[1] | a.l[2] | a.l[3] | a.l[4] | a.l[5]) == 0; }
+static fp fp_zero(void) { fp r; memset(&r, 0, sizeof r); return r; }
+static fp fp_neg(fp a) { return fp_is_zero(a) ? a : fp_sub(fp_zero(), a); }
+static fp fp_dbl(fp a) { return fp_add(a, a); }
+
+/* CIOS Montgomery multiplication. */
+static fp fp_mul(fp a, fp b) {
+  uint64_t t[8] = {0};
+  for (int i = 0; i < 6; ++i) {
+    uint64_t c = 0;
+    for (int j = 0; j < 6; ++j) {
+      u128 x = (u128)a.l[j] * b.l[i] + t[j] + c;
+      t[j] = (uint64_t)x;
+      c = (uint64_t)(x >> 64);
+    }
+    u128 s = (u128)t[6] + c;
+    t[6] = (uint64_t)s;
+    t[7] = (uint64_t)(s >> 64);
+    uint64_t m = t[0] * PINV;
+    u128 x = (u128)m * P.l[0] + t[0];
+    c = (uint64_t)(x >> 64);
+    for (int j = 1; j < 6; ++j) {
+      x = (u128)m * P.l[j] + t[j] + c;
+      t[j - 1] = (uint64_t)x;
+      c = (uint64_t)(x >> 64);
+    }
+    s = (u128)t[6] + c;
+    t[5] = (uint64_t)s;
+    t[6] = t[7] + (uint64_t)(s >> 64);
+  }
+  fp r;
+  memcpy(r.l, t, 48);
+  if (t[6] || fp_geq(&r, &P)) fp_sub_raw(&r, &r, &P);
+  return r;
+}
+
+static fp2 fp2_add(fp2 a, fp2 b) { fp2 r = {fp_add(a.c0, b.c0), fp_add(a.c1, b.c1)}; return r; }
+static fp2 fp2_sub(fp2 a, fp2 b) { fp2 r = {fp_sub(a.c0, b.c0), fp_sub(a.c1, b.c1)}; return r; }
+static fp2 fp2_neg(fp2 a) { fp2 r = {fp_neg(a.c0), fp_neg(a.c1)}; return r; }
+static fp2 fp2_dbl(fp2 a) { return fp2_add(a, a); }
+static int fp2_is_zero(fp2 a) { return fp_is_zero(a.c0) && fp_is_zero(a.c1); }
+static fp2 fp2_mul(fp2 a, fp2 b) {
+  fp t0 = fp_mul(a.c0, b.c0), t1 = fp_mul(a.c1, b.c1);
+  fp t2 = fp_mul(fp_add(a.c0, a.c1), fp_add(b.c0, b.c1));
+  fp2 r = {fp_sub(t0, t1), fp_sub(t2, fp_add(t0, t1))};
+  return r;
+}
+
+/* ---- Jacobian group law, generated for Fp (G1) and Fp2 (G2) ---------------------------- */
+#define DEFINE_CURVE(PFX, F, F_ADD, F_SUB, F_MUL, F_DBL, F_NEG, F_ISZ, F_ONE, F_ZERO)          \
+  typedef struct { F x, y, z; } PFX##_jac;                                                   \
+  typedef struct { F x, y; } PFX##_aff;                                                      \
+  static PFX##_jac PFX##_identity(void) { PFX##_jac r = {F_ONE, F_ONE, F_ZERO}; return r; }  \
+  static int PFX##_is_identity(const PFX##_jac* p) { return F_ISZ(p->z); }                   \
+  static int PFX##_aff_is_identity(const PFX##_aff* p) { return F_ISZ(p->x) && F_ISZ(p->y); } \
+  static PFX##_jac PFX##_dbl(PFX##_jac p) {                                                  \
+    if (F_ISZ(p.z)) return p;                                                                \
+    F A = F_MUL(p.x, p.x), B = F_MUL(p.y, p.y), C = F_MUL(B, B);                             \
+    F xb = F_ADD(p.x, B);                                                                    \
+    F D = F_DBL(F_SUB(F_MUL(xb, xb), F_ADD(A, C)));                                          \
+    F E = F_ADD(F_DBL(A), A);                                                                \
+    PFX##_jac r;                                                                             \
+    r.x = F_SUB(F_MUL(E, E), F_DBL(D));                                                      \
+    r.y = F_SUB(F_MUL(E, F_SUB(D, r.x)), F_DBL(F_DBL(F_DBL(C))));                            \
+    r.z = F_DBL(F_MUL(p.y, p.z));                                                            \
+    return r;                                                                                \
+  }                                                                                          \
+  static PFX##_jac PFX##_add(PFX##_jac p, PFX##_jac q) {                                     \
+    if (F_ISZ(p.z)) return q;                                                                \
+    if (F_ISZ(q.z)) return p;                                                                \
+    F Z1Z1 = F_MUL(p.z, p.z), Z2Z2 = F_MUL(q.z, q.z);                                        \
+    F U1 = F_MUL(p.x, Z2Z2), U2 = F_MUL(q.x, Z1Z1);                                          \
+    F S1 = F_MUL(F_MUL(p.y, q.z), Z2Z2), S2 = F_MUL(F_MUL(q.y, p.z), Z1Z1);                  \
+    F H = F_SUB(U2, U1), R = F_SUB(S2, S1);                                                  \
+    if (F_ISZ(H)) {                                                                          \
+      if (F_ISZ(R)) return PFX##_dbl(p);                                                     \
+      return PFX##_identity();                                                               \
+    }                                                                                        \
+    F H2 = F_DBL(H);                                                                         \
+    F I = F_MUL(H2, H2), J = F_MUL(H, I), r2 = F_DBL(R), V = F_MUL(U1, I);                   \
+    PFX##_jac o;                                                                             \
+    o.x = F_SUB(F_SUB(F_MUL(r2, r2), J), F_DBL(V));                                          \
+    o.y = F_SUB(F_MUL(r2, F_SUB(V, o.x)), F_DBL(F_MUL(S1, J)));                              \
+    F zz = F_ADD(p.z, q.z);                                                                  \
+    o.z = F_MUL(F_SUB(F_MUL(zz, zz), F_ADD(Z1Z1, Z2Z2)), H);                                 \
+    return o;                                                                                \
+  }                                                                                          \
+  static PFX##_jac PFX##_add_aff(PFX##_jac p, PFX##_aff q) {                                 \
+    if (PFX##_aff_is_identity(&q)) return p;                                                 \
+    if (F_ISZ(p.z)) { PFX##_jac r = {q.x, q.y, F_ONE}; return r; }                           \
+    F Z1Z1 = F_MUL(p.z, p.z);                                                                \
+    F U2 = F_MUL(q.x, Z1Z1), S2 = F_MUL(F_MUL(q.y, p.z), Z1Z1);                              \
+    F H = F_SUB(U2, p.x), R = F_SUB(S2, p.y);                                                \
+    if (F_ISZ(H)) {                                                                          \
+      if (F_ISZ(R)) return PFX##_dbl(p);                                                     \
+      return PFX##_identity();                                                               \
+    }                                                                                        \
+    F HH = F_MUL(H, H), I = F_DBL(F_DBL(HH)), J = F_MUL(H, I), r2 = F_DBL(R);                \
+    F V = F_MUL(p.x, I);                                                                     \
+    PFX##_jac o;                                                                             \
+    o.x = F_SUB(F_SUB(F_MUL(r2, r2), J), F_DBL(V));                                          \
+    o.y = F_SUB(F_MUL(r2, F_SUB(V, o.x)), F_DBL(F_MUL(p.y, J)));                             \
+    F zh = F_ADD(p.z, H);                                                                    \
+    o.z = F_SUB(F_SUB(F_MUL(zh, zh), Z1Z1), HH);                                             \
+    return o;                                                                                \
+  }                                                                                          \
+  static PFX##_aff PFX##_aff_neg(PFX##_aff q) {                                              \
+    if (!PFX##_aff_is_identity(&q)) q.y = F_NEG(q.y);                                        \
+    return q;                                                                                \
+  }                                                                                          \
+  /* Pippenger with signed c-bit digits; windows in parallel.  Same digit rule as the GPU. */ \
+  static void PFX##_msm(const PFX##_aff* pts, const uint8_t* sc, size_t n, int c,            \
+                        PFX##_jac* out) {                                                    \
+    int nw = (256 + c - 1) / c;                                                              \
+    uint32_t M = 1u << (c - 1);                                                              \
+    int32_t* dig = (int32_t*)malloc(sizeof(int32_t) * n * (size_t)nw);                       \
+    for (size_t i = 0; i < n; ++i) {                                                         \
+      uint32_t s[9];                                                                         \
+      memcpy(s, sc + 32 * i, 32);                                                            \
+      s[7] &= 0x7fffffffu;                                                                   \
+      s[8] = 0;                                                                              \
+      int carry = 0;                                                                         \
+      for (int w = 0; w < nw; ++w) {                                                         \
+        int bit = w * c, word = bit >> 5, off = bit & 31;                                    \
+        uint64_t two = (uint64_t)s[word] | ((uint64_t)s[word + 1] << 32);                    \
+        int d = (int)((two >> off) & ((1u << c) - 1u)) + carry;                              \
+        if (d > (int)M) { d -= (int)(1u << c); carry = 1; } else { carry = 0; }              \
+        dig[(size_t)w * n + i] = d;                                                          \
+      }                                                                                      \
+    }                                                                                        \
+    PFX##_jac* wsum = (PFX##_jac*)malloc(sizeof(PFX##_jac) * nw);                            \
+    _Pragma("omp parallel for schedule(dynamic, 1)")                                         \
+    for (int w = 0; w < nw; ++w) {                                                           \
+      PFX##_jac* bk = (PFX##_jac*)malloc(sizeof(PFX##_jac) * M);                             \
+      for (uint32_t b = 0; b < M; ++b) bk[b] = PFX##_identity();                             \
+      for (size_t i = 0; i < n; ++i) {                                                       \
+        int d = dig[(size_t)w * n + i];                                                      \
+        if (!d) continue;                                                                    \
+        PFX##_aff q = pts[i];                                                                \
+        if (d < 0) { q = PFX##_aff_neg(q); d = -d; }                                         \
+        bk[d - 1] = PFX##_add_aff(bk[d - 1], q);                                             \
+      }                                                                                      \
+      PFX##_jac run = PFX##_identity(), acc = PFX##_identity();                              \
+      for (int b = (int)M - 1; b >= 0; --b) {                                                \
+        run = PFX##_add(run, bk[b]);                                                         \
+        acc = PFX##_add(acc, run);                                                           \
+      }                                                                                      \
+      wsum[w] = acc;                                                                         \
+      free(bk);                                                                              \
+    }                                                                                        \
+    PFX##_jac acc = PFX##_identity();                                                        \
+    for (int w = nw - 1; w >= 0; --w) {                                                      \
+      for (int k = 0; k < c; ++k) acc = PFX##_dbl(acc);                                      \
+      acc = PFX##_add(acc, wsum[w]);                                                         \
+    }                                                                                        \
+    *out = acc;                                                                              \
+    free(wsum);                                                                              \
+    free(dig);                                                                               \
+  }
+
+static fp2 FP2_ONE_V(void) { fp2 r = {ONE, {{0}}}; return r; }
+static fp2 FP2_ZERO_V(void) { fp2 r; memset(&r, 0, sizeof r); return r; }
+
+DEFINE_CURVE(g1, fp, fp_add, fp_sub, fp_mul, fp_dbl, fp_neg, fp_is_zero, ONE, fp_zero())
+DEFINE_CURVE(g2, fp2, fp2_add, fp2_sub, fp2_mul, fp2_dbl, fp2_neg, fp2_is_zero, FP2_ONE_V(),
+             FP2_ZERO_V())
+
+static int default_window(size_t n) {
+  int lg = 0;
+  while (((size_t)1 << (lg + 1)) <= n) ++lg;
+  int c = lg - 4;
+  if (c < 4) c = 4;
+  if (c > 16) c = 16;
+  return c;
+}
+
+static fp fp_inv(fp a) {
+  /* a^(p-2) */
+  fp e = P;
+  e.l[0] -= 2;
+  fp r = ONE;
+  for (int i = 383; i >= 0; --i) {
+    r = fp_mul(r, r);
+    if ((e.l[i >> 6] >> (i & 63)) & 1) r = fp_mul(r, a);
+  }
+  return r;
+}
+
+/* ---- exported (ctypes) ------------------------------------------------------------------ */
+/* out[i] = (i + 1) * G1 for i < n, affine (inputs for timing samples and closed-form checks). */
+int oracle_g1_consecutive_multiples(const void* gen_affine, size_t n, void* out) {
+  g1_aff g = *(const g1_aff*)gen_affine;
+  g1_jac* tmp = (g1_jac*)malloc(sizeof(g1_jac) * n);
+  g1_jac acc = g1_identity();
+  for (size_t i = 0; i < n; ++i) {
+    acc = g1_add_aff(acc, g);
+    tmp[i] = acc;
+  }
+  g1_aff* o = (g1_aff*)out;
+  const size_t CH = 4096;
+  #pragma omp parallel for schedule(static)
+  for (size_t s = 0; s < n; s += CH) {
+    size_t e = s + CH < n ? s + CH : n;
+    fp* pre = (fp*)malloc(sizeof(fp) * (e - s));
+    fp prod = ONE;
+    for (size_t i = s; i < e; ++i) { pre[i - s] = prod; prod = fp_mul(prod, tmp[i].z); }
+    fp inv = fp_inv(prod);
+    for (size_t i = e; i-- > s;) {
+      fp zi = fp_mul(inv, pre[i - s]);
+      inv = fp_mul(inv, tmp[i].z);
+      fp zi2 = fp_mul(zi, zi);
+      o[i].x = fp_mul(tmp[i].x, zi2);
+      o[i].y = fp_mul(tmp[i].y, fp_mul(zi2, zi));
+    }
+    free(pre);
+  }
+  free(tmp);
+  return 0;
+}
+
+int oracle_g1_msm(const void* points, const uint8_t* scalars, size_t n, int c, void* out) {
+  if (c <= 0) c = default_window(n);
+  g1_msm((const g1_aff*)points, scalars, n, c, (g1_jac*)out);
+  return 0;
+}
+
+int oracle_g2_msm(const void* points, const uint8_t* scalars, size_t n, int c, void* out) {
+  if (c <= 0) c = default_window(n);
+  g2_msm((const g2_aff*)points, scalars, n, c, (g2_jac*)out);
+  return 0;
+}
+
+int oracle_fp_mul(const void* a, const void* b, void* out) {
+  *(fp*)out = fp_mul(*(const fp*)a, *(const fp*)b);
+  return 0;
+}
+
+int oracle_max_threads(void) {
+#ifdef _OPENMP
+  extern int omp_get_max_threads(void);
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

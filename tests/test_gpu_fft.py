@@ -1,0 +1,167 @@
+"""GPU parity for the Radix2Domain transforms (weighted_btx/src/fft.rs, weighted_pfe/src/fft.rs):
+G1/G2 group FFTs against the closed form DFT_Fr(s) * G, GT transforms against gT^DFT(s), and
+cyclotomic transforms against the oracle's restatement bit-exactly (raw values, before any
+hard exponentiation, are determined by the base-|x| decomposition both sides use)."""
+import random
+
+import pytest
+
+from helpers import O, g1_mul_gen, g2_mul_gen
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api(dev):
+    from weighted_bte_amd import api as A
+    return A
+
+
+def g1_jac_bytes(aff_list):
+    return b"".join(O.g1_jac_to_bytes(O.G1.from_affine(a)) for a in aff_list)
+
+
+def g2_jac_bytes(aff_list):
+    return b"".join(O.g2_jac_to_bytes(O.G2.from_affine(a)) for a in aff_list)
+
+
+def read_g1(api, t, n):
+    aff = api.to_host(api.batch_normalize_g1(t))
+    return [O.g1_affine_from_bytes(aff[96 * i: 96 * i + 96]) for i in range(n)]
+
+
+def read_g2(api, t, n):
+    aff = api.to_host(api.batch_normalize_g2(t))
+    return [O.g2_affine_from_bytes(aff[192 * i: 192 * i + 192]) for i in range(n)]
+
+
+def read_f12(api, t, n):
+    b = api.to_host(t)
+    return [O.f12_from_bytes(b[576 * i: 576 * i + 576]) for i in range(n)]
+
+
+def f12_bytes(vals):
+    return b"".join(O.f12_to_bytes(v) for v in vals)
+
+
+def rand_cyclotomic(rnd):
+    f = tuple(tuple((rnd.randrange(O.P), rnd.randrange(O.P)) for _ in range(3)) for _ in range(2))
+    return O.easy_final_exponentiation(f)
+
+
+@pytest.mark.parametrize("n", [1, 2, 8, 32])
+def test_g1_fft_closed_form_and_roundtrip(api, n):
+    rnd = random.Random(10 + n)
+    s = [rnd.randrange(O.R) for _ in range(n)]
+    s[0] = 0  # an identity input
+    x = api.to_device(g1_jac_bytes([g1_mul_gen(k) for k in s]))
+    d = api.Radix2Domain(n)
+    y = d.fft_g1(x)
+    want = O.fr_fft(s)
+    assert read_g1(api, y, n) == [g1_mul_gen(k) for k in want]
+    back = d.ifft_g1(y)
+    assert read_g1(api, back, n) == [g1_mul_gen(k) for k in s]
+    unscaled = d.ifft_g1(y, scaled=False)
+    assert read_g1(api, unscaled, n) == [g1_mul_gen(k * n) for k in s]
+
+
+def test_g1_fft_matches_literal_oracle_fft(api):
+    rnd = random.Random(3)
+    n = 8
+    pts = [O.G1.mul(O.g1_gen_jac(), rnd.randrange(O.R)) for _ in range(n)]
+    x = api.to_device(b"".join(O.g1_jac_to_bytes(p) for p in pts))
+    got = read_g1(api, api.Radix2Domain(n).fft_g1(x), n)
+    assert got == [O.G1.to_affine(v) for v in O.group_fft(O.G1, pts)]
+
+
+@pytest.mark.parametrize("n", [2, 8])
+def test_g2_fft_closed_form_and_roundtrip(api, n):
+    rnd = random.Random(20 + n)
+    s = [rnd.randrange(O.R) for _ in range(n)]
+    x = api.to_device(g2_jac_bytes([g2_mul_gen(k) for k in s]))
+    d = api.Radix2Domain(n)
+    y = d.fft_g2(x)
+    assert read_g2(api, y, n) == [g2_mul_gen(k) for k in O.fr_fft(s)]
+    assert read_g2(api, d.ifft_g2(y), n) == [g2_mul_gen(k) for k in s]
+
+
+def test_fft_invalid_sizes(api):
+    with pytest.raises(ValueError):
+        api.Radix2Domain(3)
+    from weighted_bte_amd import BteError, call
+    import ctypes
+    buf = api.to_device(bytes(144 * 3))
+    with pytest.raises(BteError):
+        call("bte_fft_g1", ctypes.c_void_p(buf.data_ptr()), 3, 0, 0, ctypes.c_void_p(0))
+
+
+def test_g1_fft_2_8_closed_form(api):
+    """Config-3 G1 size 2^8: inputs s_i * G, outputs DFT_Fr(s) * G."""
+    n = 256
+    rng = O.SeededRng(3, 8)
+    s = [rng.scalar() for _ in range(n)]
+    x = api.g1_generator_mul(api.to_device(b"".join(O.scalar_to_le32(k) for k in s)))
+    # affine -> Jacobian with z = 1 on the host side of the test
+    xa = api.to_host(x)
+    pts = [O.g1_affine_from_bytes(xa[96 * i: 96 * i + 96]) for i in range(n)]
+    d = api.Radix2Domain(n)
+    y = d.fft_g1(api.to_device(g1_jac_bytes(pts)))
+    got = read_g1(api, y, n)
+    want = O.fr_fft(s)
+    for k in (0, 1, 2, 77, 128, 255):
+        assert got[k] == g1_mul_gen(want[k])
+    assert read_g1(api, d.ifft_g1(y), n) == pts
+
+
+def test_gt_fft_closed_form(api):
+    rnd = random.Random(30)
+    n = 4
+    g = O.gt_generator()
+    s = [rnd.randrange(O.R) for _ in range(n)]
+    vals = [O.f12_pow(g, k) for k in s]
+    d = api.Radix2Domain(n)
+    y = d.fft_gt(api.to_device(f12_bytes(vals)))
+    assert read_f12(api, y, n) == [O.f12_pow(g, k) for k in O.fr_fft(s)]
+    assert read_f12(api, d.ifft_gt(y), n) == vals
+    unscaled = read_f12(api, d.ifft_gt(y, scaled=False), n)
+    assert unscaled == [O.f12_pow(g, k * n % O.R) for k in s]
+
+
+def test_cyclotomic_fft_raw_bit_exact(api):
+    rnd = random.Random(31)
+    n = 4
+    vals = [rand_cyclotomic(rnd) for _ in range(n)]
+    d = api.Radix2Domain(n)
+    x = api.to_device(f12_bytes(vals))
+    assert read_f12(api, d.fft_cyclotomic(x), n) == O.fft_cyclotomic(vals)
+    assert read_f12(api, d.ifft_cyclotomic_unscaled(x), n) == O.fft_cyclotomic(vals, inverse=True)
+
+
+@pytest.mark.parametrize("n,count", [(4, 2), (8, 3), (8, 4), (8, 0)])
+def test_cyclotomic_prefix_ifft(api, n, count):
+    rnd = random.Random(40 + n + count)
+    vals = [rand_cyclotomic(rnd) for _ in range(n)]
+    got = read_f12(api, api.Radix2Domain(n).ifft_cyclotomic_prefix_unscaled(
+        api.to_device(f12_bytes(vals)), count), count)
+    want = O.ifft_cyclotomic_prefix_unscaled(vals, count)
+    assert got == want[:count]
+    # after the hard part the prefix equals the unscaled plain inverse transform (fft.rs:898-957)
+    for a, b in zip(got, O.fft_cyclotomic(vals, inverse=True)[:count]):
+        assert O.hard_final_exponentiation(a) == O.hard_final_exponentiation(b)
+
+
+def test_cyclotomic_scalar_actions(api):
+    rnd = random.Random(50)
+    g = O.gt_generator()
+    scal = [0, 1, O.R - 1, O.X_ABS, rnd.randrange(O.R), rnd.randrange(O.R)]
+    vals = [rand_cyclotomic(rnd) for _ in scal[:3]] + [g, g, O.f12_pow(g, 5)]
+    got = read_f12(api, api.cyclotomic_mul_scalar_decomposed(api.to_device(f12_bytes(vals)), scal),
+                   len(vals))
+    for v, s, r in zip(vals, scal, got):
+        assert r == O.cyclotomic_mul_scalar_decomposed(v, s)
+    for i in (3, 4, 5):
+        assert got[i] == O.f12_pow(vals[i], scal[i])
+    coeffs = [0, 1, -1, 7, -1023, 511]
+    got = read_f12(api, api.mul_cyclotomic_signed_small(api.to_device(f12_bytes(vals)), coeffs),
+                   len(vals))
+    assert got == [O.mul_cyclotomic_signed_small(v, c) for v, c in zip(vals, coeffs)]

@@ -36,6 +36,20 @@ _SIGNATURES = {
     "bte_g2_batch_normalize": [_vp, _sz, _vp, _vp],
     "bte_g1_generator_mul": [_vp, _sz, _vp, _vp],
     "bte_g2_generator_mul": [_vp, _sz, _vp, _vp],
+    "bte_fft_g1": [_vp, _sz, _i, _i, _vp],
+    "bte_fft_g2": [_vp, _sz, _i, _i, _vp],
+    "bte_fft_gt": [_vp, _sz, _i, _i, _vp],
+    "bte_fft_cyclotomic": [_vp, _sz, _i, _vp],
+    "bte_ifft_cyclotomic_prefix_unscaled": [_vp, _sz, _sz, _vp],
+    "bte_cyclotomic_pow_scalars": [_vp, _vp, _sz, _vp],
+    "bte_cyclotomic_pow_small": [_vp, _vp, _sz, _vp],
+    "bte_g2_prepare_lines": [_vp, _sz, _vp, _vp],
+    "bte_miller_loop_prepared": [_vp, _vp, _sz, _vp, _vp],
+    "bte_miller_loop_pairs": [_vp, _vp, _sz, _vp, _vp],
+    "bte_multi_miller_loop": [_vp, _vp, _sz, _vp, _sz, _vp, _vp],
+    "bte_final_exp_easy": [_vp, _sz, _vp, _vp],
+    "bte_final_exp_hard": [_vp, _sz, _vp, _vp],
+    "bte_final_exp": [_vp, _sz, _vp, _vp],
     "bte_profile_enable": [_i],
     "bte_profile_read": [ctypes.c_char_p, _sz],
     "bte_probe_mac_rate": [ctypes.POINTER(ctypes.c_double), _vp],

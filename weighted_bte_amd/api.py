@@ -121,3 +121,166 @@ def g2_generator_mul(scalar_bytes):
 
 def set_msm_window(group, n, bits):
     call("bte_msm_set_window", group, n, bits)
+
+
+# --- a4..a9: Radix2Domain transforms (weighted_btx/src/fft.rs, weighted_pfe/src/fft.rs) -----
+class Radix2Domain:
+    """Radix2Domain::new (fft.rs:164-191): the size must be a nonzero power of two <= 2^32."""
+
+    def __init__(self, size):
+        if size <= 0 or size & (size - 1) or size > (1 << 32):
+            raise ValueError("InvalidBatchSize(%d)" % size)
+        self.size = size
+
+    def _check(self, values, width):
+        values = _dev(values)
+        if values.numel() != self.size * width:
+            raise ValueError("values.len() != domain size")
+        return values
+
+    def fft_g1(self, values):
+        v = self._check(values, G1_JAC).clone()
+        call("bte_fft_g1", _ptr(v), self.size, 0, 0, _stream())
+        return v
+
+    def ifft_g1(self, values, scaled=True):
+        v = self._check(values, G1_JAC).clone()
+        call("bte_fft_g1", _ptr(v), self.size, 1, 1 if scaled else 0, _stream())
+        return v
+
+    def fft_g2(self, values):
+        v = self._check(values, G2_JAC).clone()
+        call("bte_fft_g2", _ptr(v), self.size, 0, 0, _stream())
+        return v
+
+    def ifft_g2(self, values, scaled=True):
+        v = self._check(values, G2_JAC).clone()
+        call("bte_fft_g2", _ptr(v), self.size, 1, 1 if scaled else 0, _stream())
+        return v
+
+    def fft_gt(self, values):
+        v = self._check(values, FP12).clone()
+        call("bte_fft_gt", _ptr(v), self.size, 0, 0, _stream())
+        return v
+
+    def ifft_gt(self, values, scaled=True):
+        v = self._check(values, FP12).clone()
+        call("bte_fft_gt", _ptr(v), self.size, 1, 1 if scaled else 0, _stream())
+        return v
+
+    def fft_cyclotomic(self, values):
+        v = self._check(values, FP12).clone()
+        call("bte_fft_cyclotomic", _ptr(v), self.size, 0, _stream())
+        return v
+
+    def ifft_cyclotomic_unscaled(self, values):
+        v = self._check(values, FP12).clone()
+        call("bte_fft_cyclotomic", _ptr(v), self.size, 1, _stream())
+        return v
+
+    def ifft_cyclotomic_prefix_unscaled(self, values, output_count):
+        if output_count > self.size // 2:
+            raise ValueError("output_count > size / 2")
+        v = self._check(values, FP12).clone()
+        call("bte_ifft_cyclotomic_prefix_unscaled", _ptr(v), self.size, output_count, _stream())
+        return v[: output_count * FP12]
+
+
+def cyclotomic_mul_scalar_decomposed(values, scalars):
+    """fft.rs:65-71, element-wise; scalars are canonical ints (host)."""
+    v = _dev(values).clone()
+    n = v.numel() // FP12
+    if len(scalars) != n:
+        raise ValueError("scalars.len() != values.len()")
+    hs = b"".join(int(s).to_bytes(32, "little") for s in scalars)
+    buf = ctypes.create_string_buffer(hs, len(hs))
+    call("bte_cyclotomic_pow_scalars", _ptr(v), buf, n, _stream())
+    torch.cuda.current_stream().synchronize()
+    return v
+
+
+def mul_cyclotomic_signed_small(values, coefficients):
+    """weighted_pfe/src/fft.rs:671-697, element-wise."""
+    v = _dev(values).clone()
+    n = v.numel() // FP12
+    c = torch.tensor(list(coefficients), dtype=torch.int64, device=v.device)
+    if c.numel() != n:
+        raise ValueError("coefficients.len() != values.len()")
+    call("bte_cyclotomic_pow_small", _ptr(v), _ptr(c), n, _stream())
+    return v
+
+
+# --- a10..a14: pairing (weighted_btx/src/final_exponentiation.rs) -----------------------------
+MILLER_LINE_COUNT = 68
+LINES_BYTES = MILLER_LINE_COUNT * 288
+
+
+def prepare_g2_lines(q_affine):
+    """PreparedG2Lines::from_affine for every point (final_exponentiation.rs:186-195)."""
+    q = _dev(q_affine)
+    n = q.numel() // G2_AFFINE
+    out = _empty(n * LINES_BYTES, q)
+    call("bte_g2_prepare_lines", _ptr(q), n, _ptr(out), _stream())
+    return out
+
+
+def miller_loop_prepared(lines, p_affine):
+    lines, p = _dev(lines), _dev(p_affine)
+    n = p.numel() // G1_AFFINE
+    if lines.numel() != n * LINES_BYTES:
+        raise ValueError("lines.len() != points.len()")
+    out = _empty(n * FP12, p)
+    call("bte_miller_loop_prepared", _ptr(lines), _ptr(p), n, _ptr(out), _stream())
+    return out
+
+
+def unprepared_multi_miller_loop(left, right, seg_offsets=None):
+    """final_exponentiation.rs:229-273; with seg_offsets (a list, len nseg+1) it evaluates one
+    multi-Miller loop per segment of pairs (uneven arities)."""
+    left, right = _dev(left), _dev(right)
+    npairs = left.numel() // G1_AFFINE
+    if right.numel() != npairs * G2_AFFINE:
+        raise ValueError("left.len() != right.len()")
+    if seg_offsets is None:
+        seg_offsets = [0, npairs]
+    if seg_offsets[0] != 0 or seg_offsets[-1] != npairs or any(
+            b < a for a, b in zip(seg_offsets, seg_offsets[1:])):
+        raise ValueError("bad segment offsets")
+    offs = torch.tensor(seg_offsets, dtype=torch.int64, device=left.device)
+    nseg = len(seg_offsets) - 1
+    out = _empty(nseg * FP12, left)
+    call("bte_multi_miller_loop", _ptr(left), _ptr(right), npairs, _ptr(offs), nseg, _ptr(out),
+         _stream())
+    return out
+
+
+def miller_loop_pairs(left, right):
+    left, right = _dev(left), _dev(right)
+    n = left.numel() // G1_AFFINE
+    out = _empty(n * FP12, left)
+    call("bte_miller_loop_pairs", _ptr(left), _ptr(right), n, _ptr(out), _stream())
+    return out
+
+
+def easy_final_exponentiation(values):
+    v = _dev(values)
+    out = torch.empty_like(v)
+    call("bte_final_exp_easy", _ptr(v), v.numel() // FP12, _ptr(out), _stream())
+    return out
+
+
+batch_easy_final_exponentiation = easy_final_exponentiation
+
+
+def hard_final_exponentiation(values):
+    v = _dev(values)
+    out = torch.empty_like(v)
+    call("bte_final_exp_hard", _ptr(v), v.numel() // FP12, _ptr(out), _stream())
+    return out
+
+
+def full_final_exponentiation(values):
+    v = _dev(values)
+    out = torch.empty_like(v)
+    call("bte_final_exp", _ptr(v), v.numel() // FP12, _ptr(out), _stream())
+    return out

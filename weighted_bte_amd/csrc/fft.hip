@@ -1,0 +1,352 @@
+// Radix-2 FFTs over the Fr domain for G1, G2, GT and the cyclotomic subgroup (gfx950).
+//
+// Semantics of weighted_btx/src/fft.rs:156-390 (Radix2Domain): the natural-order transform
+// X_k = sum_j x_j w^(jk) with w = ROOT_OF_UNITY^(2^(32 - log n)) (w^-1 for the inverse), computed
+// as a bit-reversal permutation followed by log n decimation-in-time stages; butterfly
+// (u, v) -> (u + v t, u - v t) with the group's scalar action for "v t"; the scaled inverse
+// multiplies every output by n^-1.  For GT and cyclotomic values the group law is Fp12
+// multiplication, the inverse is conjugation and the scalar action is the decomposed plan of
+// cyclo.hpp (fft.rs:437-526, weighted_pfe/src/fft.rs:324-365).
+//
+// Every stage is one launch with one thread per butterfly; twiddles w^j (j < n/2) and their
+// cyclotomic plans are built once per (size, direction) on the host and kept in device memory.
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "curve.hpp"
+#include "cyclo.hpp"
+#include "host_field.hpp"
+
+namespace bte {
+
+// ---------------------------------------------------------------------------------------------
+// Host: domain twiddles and cyclotomic scalar plans
+// ---------------------------------------------------------------------------------------------
+static const uint64_t X_ABS = 0xd201000000010000ull;
+
+// weighted_btx/src/fft.rs:543-567: canonical scalar -> four base-|x| digits.
+static void decompose_base_x(const bteh::Fr& s, uint64_t out[4]) {
+  uint64_t limbs[4] = {s.l[0], s.l[1], s.l[2], s.l[3]};
+  for (int c = 0; c < 4; ++c) {
+    unsigned __int128 rem = 0;
+    for (int i = 3; i >= 0; --i) {
+      unsigned __int128 d = (rem << 64) | limbs[i];
+      limbs[i] = (uint64_t)(d / X_ABS);
+      rem = d % X_ABS;
+    }
+    out[c] = (uint64_t)rem;
+  }
+}
+
+// weighted_btx/src/fft.rs:569-594: width-4 NAF of a u64 (65 digits for the carry into bit 64).
+static int wnaf4(uint64_t value, int8_t digits[CYC_DIGITS]) {
+  unsigned __int128 v = value;
+  int pos = 0;
+  for (int i = 0; i < CYC_DIGITS; ++i) digits[i] = 0;
+  while (v != 0) {
+    if (v & 1) {
+      int d = (int)(v & 15);
+      if (d >= 8) d -= 16;
+      digits[pos] = (int8_t)d;
+      if (d > 0) v -= (unsigned)d; else v += (unsigned)(-d);
+    }
+    v >>= 1;
+    ++pos;
+  }
+  return pos;
+}
+
+static cyc_plan make_plan(const bteh::Fr& s) {
+  cyc_plan p;
+  memset(&p, 0, sizeof(p));
+  uint64_t comps[4];
+  decompose_base_x(s, comps);
+  int count = 0;
+  for (int c = 0; c < 4; ++c) {
+    int n = wnaf4(comps[c], p.naf[c]);
+    if (n > count) count = n;
+  }
+  p.count = (uint8_t)count;
+  return p;
+}
+
+// fft.rs:164-191: generator of the size-2^logn subgroup (or its inverse).
+static bteh::Fr domain_generator(int logn, bool inverse) {
+  bteh::Fr g;
+  memcpy(g.l, bteh::FR_ROOT_OF_UNITY, 32);
+  for (int i = logn; i < bteh::FR_S; ++i) g = bteh::fr_mul(g, g);
+  return inverse ? bteh::fr_inv(g) : g;
+}
+
+struct DomainTables {
+  uint32_t* twiddles = nullptr;  // n/2 canonical scalars (8 x u32 each): w^j
+  cyc_plan* plans = nullptr;     // n/2 plans of w^j
+  uint32_t* size_inv = nullptr;  // n^-1 (8 x u32)
+  cyc_plan* size_inv_plan = nullptr;
+};
+
+static std::mutex g_dom_mu;
+static std::map<std::tuple<int, int, int>, DomainTables> g_domains;  // (device, logn, inverse)
+
+static int domain_tables(int logn, bool inverse, bool want_plans, DomainTables** out, hipStream_t s) {
+  int dev = 0;
+  BTE_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_dom_mu);
+  DomainTables& t = g_domains[std::make_tuple(dev, logn, inverse ? 1 : 0)];
+  const size_t half = logn ? ((size_t)1 << (logn - 1)) : 0;
+  if (!t.twiddles) {
+    std::vector<bteh::Fr> tw(half ? half : 1);
+    bteh::Fr w = domain_generator(logn, inverse);
+    tw[0] = bteh::fr_from_u64(1);
+    for (size_t j = 1; j < half; ++j) tw[j] = bteh::fr_mul(tw[j - 1], w);
+    bteh::Fr ninv = bteh::fr_inv(bteh::fr_from_u64((uint64_t)1 << logn));
+    BTE_HIP(hipMalloc((void**)&t.twiddles, tw.size() * 32));
+    BTE_HIP(hipMemcpy(t.twiddles, tw.data(), tw.size() * 32, hipMemcpyHostToDevice));
+    BTE_HIP(hipMalloc((void**)&t.size_inv, 32));
+    BTE_HIP(hipMemcpy(t.size_inv, ninv.l, 32, hipMemcpyHostToDevice));
+  }
+  if (want_plans && !t.plans) {
+    std::vector<cyc_plan> pl(half ? half : 1);
+    bteh::Fr w = domain_generator(logn, inverse);
+    bteh::Fr cur = bteh::fr_from_u64(1);
+    for (size_t j = 0; j < half; ++j) {
+      pl[j] = make_plan(cur);
+      cur = bteh::fr_mul(cur, w);
+    }
+    cyc_plan np = make_plan(bteh::fr_inv(bteh::fr_from_u64((uint64_t)1 << logn)));
+    BTE_HIP(hipMalloc((void**)&t.plans, pl.size() * sizeof(cyc_plan)));
+    BTE_HIP(hipMemcpy(t.plans, pl.data(), pl.size() * sizeof(cyc_plan), hipMemcpyHostToDevice));
+    BTE_HIP(hipMalloc((void**)&t.size_inv_plan, sizeof(cyc_plan)));
+    BTE_HIP(hipMemcpy(t.size_inv_plan, &np, sizeof(cyc_plan), hipMemcpyHostToDevice));
+  }
+  (void)s;
+  *out = &t;
+  return BTE_OK;
+}
+
+static int log2_exact(size_t n) {
+  if (n == 0 || (n & (n - 1))) return -1;
+  int l = 0;
+  while (((size_t)1 << l) < n) ++l;
+  return l > bteh::FR_S ? -1 : l;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------------------------
+BTE_DEV size_t bitrev(size_t i, int bits) {
+  return bits ? (size_t)(__builtin_bitreverse64((uint64_t)i) >> (64 - bits)) : 0;
+}
+
+// fft.rs:621-632
+template <class T> __global__ void k_bit_reverse(T* v, size_t n, int logn) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t r = bitrev(i, logn);
+  if (i < r) {
+    T a = v[i];
+    v[i] = v[r];
+    v[r] = a;
+  }
+}
+
+// Left-to-right double-and-add by a canonical scalar (8 x u32, below 2^255).
+template <class F> BTE_DEV jac<F> group_scalar_mul(const jac<F>& p, const uint32_t* __restrict__ k) {
+  jac<F> acc = jac_identity<F>();
+  int top = 255;
+  while (top >= 0 && !((k[top >> 5] >> (top & 31)) & 1u)) --top;
+#pragma unroll 1
+  for (int i = top; i >= 0; --i) {
+    acc = jac_dbl(acc);
+    if ((k[i >> 5] >> (i & 31)) & 1u) acc = jac_add(acc, p);
+  }
+  return acc;
+}
+
+// One DIT stage for butterfly size 2 * 2^log_half; twiddle index k << (logn - 1 - log_half).
+template <class F>
+__global__ void __launch_bounds__(64) k_group_stage(jac<F>* __restrict__ v, size_t n, int logn, int log_half,
+                                                    const uint32_t* __restrict__ tw) {
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= n / 2) return;
+  const size_t half = (size_t)1 << log_half;
+  const size_t k = t & (half - 1);
+  const size_t i = ((t >> log_half) << (log_half + 1)) + k;
+  const jac<F> u = v[i];
+  jac<F> w = v[i + half];
+  if (k) w = group_scalar_mul(w, tw + 8 * (k << (logn - 1 - log_half)));
+  v[i] = jac_add(u, w);
+  v[i + half] = jac_sub(u, w);
+}
+
+template <class F>
+__global__ void __launch_bounds__(64) k_group_scale(jac<F>* __restrict__ v, size_t n, const uint32_t* __restrict__ s) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  v[i] = group_scalar_mul(v[i], s);
+}
+
+__global__ void __launch_bounds__(64) k_cyc_stage(fp12* __restrict__ v, size_t n, int logn, int log_half,
+                                                  const cyc_plan* __restrict__ plans) {
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= n / 2) return;
+  const size_t half = (size_t)1 << log_half;
+  const size_t k = t & (half - 1);
+  const size_t i = ((t >> log_half) << (log_half + 1)) + k;
+  const fp12 u = v[i];
+  fp12 w = v[i + half];
+  if (k) w = cyclo_pow_plan(w, plans + (k << (logn - 1 - log_half)));
+  v[i] = fp12_mul(u, w);
+  v[i + half] = fp12_mul(u, fp12_conj(w));
+}
+
+// Last stage of the prefix inverse transform (fft.rs:322-349): only outputs k < count.
+__global__ void __launch_bounds__(64) k_cyc_prefix_last(fp12* __restrict__ v, size_t n, size_t count,
+                                                        const cyc_plan* __restrict__ plans) {
+  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const size_t half = n / 2;
+  fp12 w = v[half + k];
+  if (k) w = cyclo_pow_plan(w, plans + k);
+  v[k] = fp12_mul(v[k], w);
+}
+
+__global__ void __launch_bounds__(64) k_cyc_scale(fp12* __restrict__ v, size_t n, const cyc_plan* __restrict__ plan) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  v[i] = cyclo_pow_plan(v[i], plan);
+}
+
+__global__ void __launch_bounds__(64) k_cyc_pow_small(fp12* __restrict__ v, size_t n, const int64_t* __restrict__ c) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  v[i] = cyclo_pow_small(v[i], c[i]);
+}
+
+__global__ void __launch_bounds__(64) k_cyc_pow_plans(fp12* __restrict__ v, size_t n, const cyc_plan* __restrict__ plans) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  v[i] = cyclo_pow_plan(v[i], plans + i);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Drivers
+// ---------------------------------------------------------------------------------------------
+template <class F> static int group_fft(jac<F>* v, size_t n, bool inverse, bool scaled, hipStream_t s) {
+  const int logn = log2_exact(n);
+  if (logn < 0) return set_error(BTE_EINVAL, "fft: InvalidBatchSize(%zu)", n);
+  if (!v) return set_error(BTE_EINVAL, "fft: null values");
+  if (n == 1) return BTE_OK;  // a one-point transform is the identity map (n^-1 = 1)
+  DomainTables* t;
+  int rc = domain_tables(logn, inverse, false, &t, s);
+  if (rc) return rc;
+  PhaseTimer pt(s);
+  pt.begin();
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse<jac<F>>), dim3(blocks_for(n, 256)), dim3(256), 0, s, v, n, logn);
+  for (int lh = 0; lh < logn; ++lh)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage<F>), dim3(blocks_for(n / 2, 64)), dim3(64), 0, s, v, n, logn,
+                       lh, (const uint32_t*)t->twiddles);
+  pt.mark("stages");
+  if (inverse && scaled)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_scale<F>), dim3(blocks_for(n, 64)), dim3(64), 0, s, v, n,
+                       (const uint32_t*)t->size_inv);
+  pt.mark("scale");
+  BTE_CHECK_LAUNCH();
+  if (pt.on) {
+    BTE_HIP(hipStreamSynchronize(s));
+    pt.publish();
+  }
+  return BTE_OK;
+}
+
+// mode 0: full transform; mode 1: prefix inverse (last stage only for `count` outputs)
+static int cyclotomic_fft(fp12* v, size_t n, bool inverse, bool scaled, size_t count, int mode, hipStream_t s) {
+  const int logn = log2_exact(n);
+  if (logn < 0) return set_error(BTE_EINVAL, "fft: InvalidBatchSize(%zu)", n);
+  if (!v) return set_error(BTE_EINVAL, "fft: null values");
+  if (mode == 1 && count > n / 2) return set_error(BTE_EINVAL, "prefix fft: output_count > n/2");
+  if (n == 1) return BTE_OK;
+  DomainTables* t;
+  int rc = domain_tables(logn, inverse, true, &t, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse<fp12>), dim3(blocks_for(n, 256)), dim3(256), 0, s, v, n, logn);
+  const int full_stages = mode == 1 ? logn - 1 : logn;
+  for (int lh = 0; lh < full_stages; ++lh)
+    hipLaunchKernelGGL(k_cyc_stage, dim3(blocks_for(n / 2, 64)), dim3(64), 0, s, v, n, logn, lh,
+                       (const cyc_plan*)t->plans);
+  if (mode == 1 && count)
+    hipLaunchKernelGGL(k_cyc_prefix_last, dim3(blocks_for(count, 64)), dim3(64), 0, s, v, n, count,
+                       (const cyc_plan*)t->plans);
+  if (mode == 0 && inverse && scaled)
+    hipLaunchKernelGGL(k_cyc_scale, dim3(blocks_for(n, 64)), dim3(64), 0, s, v, n,
+                       (const cyc_plan*)t->size_inv_plan);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+// Plans for caller-given canonical scalars, uploaded to a stream-ordered buffer.
+static int upload_plans(const uint8_t* host_scalars, size_t n, cyc_plan** out, hipStream_t s) {
+  std::vector<cyc_plan> pl(n);
+  for (size_t i = 0; i < n; ++i) {
+    bteh::Fr f;
+    memcpy(f.l, host_scalars + 32 * i, 32);
+    if (bteh::geq<4>(f.l, bteh::FR_R)) return set_error(BTE_EINVAL, "scalar %zu is not canonical", i);
+    pl[i] = make_plan(f);
+  }
+  BTE_HIP(hipMallocAsync((void**)out, n * sizeof(cyc_plan), s));
+  BTE_HIP(hipMemcpyAsync(*out, pl.data(), n * sizeof(cyc_plan), hipMemcpyHostToDevice, s));
+  BTE_HIP(hipStreamSynchronize(s));  // pl is a host temporary
+  return BTE_OK;
+}
+
+}  // namespace bte
+
+using namespace bte;
+
+extern "C" int bte_fft_g1(bte_p1* values, size_t n, int inverse, int scaled, void* stream) {
+  return group_fft<fp>((jac<fp>*)values, n, inverse != 0, scaled != 0, (hipStream_t)stream);
+}
+
+extern "C" int bte_fft_g2(bte_p2* values, size_t n, int inverse, int scaled, void* stream) {
+  return group_fft<fp2>((jac<fp2>*)values, n, inverse != 0, scaled != 0, (hipStream_t)stream);
+}
+
+extern "C" int bte_fft_gt(bte_fp12* values, size_t n, int inverse, int scaled, void* stream) {
+  return cyclotomic_fft((fp12*)values, n, inverse != 0, scaled != 0, 0, 0, (hipStream_t)stream);
+}
+
+extern "C" int bte_fft_cyclotomic(bte_fp12* values, size_t n, int inverse, void* stream) {
+  return cyclotomic_fft((fp12*)values, n, inverse != 0, false, 0, 0, (hipStream_t)stream);
+}
+
+extern "C" int bte_ifft_cyclotomic_prefix_unscaled(bte_fp12* values, size_t n, size_t output_count,
+                                                   void* stream) {
+  return cyclotomic_fft((fp12*)values, n, true, false, output_count, 1, (hipStream_t)stream);
+}
+
+extern "C" int bte_cyclotomic_pow_scalars(bte_fp12* values, const uint8_t* host_scalars, size_t n,
+                                          void* stream) {
+  if (n == 0) return BTE_OK;
+  if (!values || !host_scalars) return set_error(BTE_EINVAL, "cyclotomic_pow_scalars: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  cyc_plan* plans;
+  int rc = upload_plans(host_scalars, n, &plans, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_cyc_pow_plans, dim3(blocks_for(n, 64)), dim3(64), 0, s, (fp12*)values, n,
+                     (const cyc_plan*)plans);
+  (void)hipFreeAsync(plans, s);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+extern "C" int bte_cyclotomic_pow_small(bte_fp12* values, const int64_t* coefficients, size_t n,
+                                        void* stream) {
+  if (n == 0) return BTE_OK;
+  if (!values || !coefficients) return set_error(BTE_EINVAL, "cyclotomic_pow_small: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_cyc_pow_small, dim3(blocks_for(n, 64)), dim3(64), 0, s, (fp12*)values, n, coefficients);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}

@@ -13,11 +13,6 @@ typedef unsigned __int128 u128;
 // ---------------------------------------------------------------------------------------------
 // Generic Montgomery arithmetic over N 64-bit limbs.
 // ---------------------------------------------------------------------------------------------
-template <int N> struct Mod {
-  const uint64_t* p;
-  uint64_t pinv;  // -p^-1 mod 2^64
-};
-
 template <int N> inline bool geq(const uint64_t* a, const uint64_t* b) {
   for (int i = N - 1; i >= 0; --i) {
     if (a[i] != b[i]) return a[i] > b[i];
@@ -44,21 +39,21 @@ template <int N> inline uint64_t add_n(uint64_t* r, const uint64_t* a, const uin
   return c;
 }
 
+// CIOS Montgomery product for moduli below 2^(64N - 2): the running value stays below 2p, so
+// the accumulator needs one extra word and a single final conditional subtraction.
 template <int N> inline void mont_mul(uint64_t* r, const uint64_t* a, const uint64_t* b,
                                       const uint64_t* p, uint64_t pinv) {
-  uint64_t t[N + 2];
-  memset(t, 0, sizeof(t));
+  uint64_t t[N + 1] = {0};
   for (int i = 0; i < N; ++i) {
+    const uint64_t bi = b[i];
     uint64_t c = 0;
     for (int j = 0; j < N; ++j) {
-      u128 x = (u128)a[j] * b[i] + t[j] + c;
+      u128 x = (u128)a[j] * bi + t[j] + c;
       t[j] = (uint64_t)x;
       c = (uint64_t)(x >> 64);
     }
-    u128 s = (u128)t[N] + c;
-    t[N] = (uint64_t)s;
-    t[N + 1] = (uint64_t)(s >> 64);
-    uint64_t m = t[0] * pinv;
+    const uint64_t tn = t[N] + c;
+    const uint64_t m = t[0] * pinv;
     u128 x = (u128)m * p[0] + t[0];
     c = (uint64_t)(x >> 64);
     for (int j = 1; j < N; ++j) {
@@ -66,15 +61,18 @@ template <int N> inline void mont_mul(uint64_t* r, const uint64_t* a, const uint
       t[j - 1] = (uint64_t)x;
       c = (uint64_t)(x >> 64);
     }
-    s = (u128)t[N] + c;
-    t[N - 1] = (uint64_t)s;
-    t[N] = t[N + 1] + (uint64_t)(s >> 64);
+    t[N - 1] = tn + c;
+    t[N] = 0;
   }
-  if (t[N] || geq<N>(t, p)) {
-    sub_n<N>(r, t, p);
-  } else {
-    memcpy(r, t, N * 8);
+  uint64_t d[N];
+  uint64_t borrow = 0;
+  for (int i = 0; i < N; ++i) {
+    u128 v = (u128)t[i] - p[i] - borrow;
+    d[i] = (uint64_t)v;
+    borrow = (uint64_t)(v >> 64) & 1;
   }
+  const uint64_t keep = 0 - borrow;  // all ones when t < p
+  for (int i = 0; i < N; ++i) r[i] = (t[i] & keep) | (d[i] & ~keep);
 }
 
 // ---------------------------------------------------------------------------------------------

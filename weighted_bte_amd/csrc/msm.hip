@@ -16,7 +16,8 @@
 //                cross chunk boundaries go to per-chunk head/tail slots.  Chunking keeps the
 //                work per thread bounded however skewed the buckets are (sparse scalars pile
 //                onto power-of-two buckets).
-//   5. fixup     one thread per bucket that crosses chunks adds its pieces
+//   5. fixup     one thread per bucket that crosses a few chunks adds its pieces; buckets that
+//                cross many (sparse-scalar pile-ups) get a 64-thread workgroup and a tree
 //   6. segsum    per window, segments of L buckets: running sums give
 //                acc_j = sum_k (k+1) B_{jL+k} and run_j = sum_k B_{jL+k}
 //   7. window sums  per window one workgroup tree-sums A = sum_j acc_j and, for each bit b of
@@ -25,6 +26,7 @@
 //   8. host      sum_w 2^(c w) T_w by one Horner pass over at most 256 doublings; this serial
 //                tail is ~0.1 ms on a host core and would cost milliseconds in one GPU lane.
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -242,10 +244,15 @@ __global__ void __launch_bounds__(64) k_msm_accumulate(const aff<F>* __restrict_
   emit();
 }
 
+// Buckets spanning more than FIX_SERIAL chunks (the sparse-scalar pile-ups) are queued for
+// k_msm_fixup_big instead of being summed by one thread.
+static constexpr uint32_t FIX_SERIAL = 8;
+
 template <class F>
 __global__ void __launch_bounds__(64) k_msm_fixup(const uint32_t* __restrict__ offsets, uint32_t nb,
-                            jac<F>* __restrict__ buckets, const jac<F>* __restrict__ heads,
-                            const jac<F>* __restrict__ tails) {
+                                                  jac<F>* __restrict__ buckets, const jac<F>* __restrict__ heads,
+                                                  const jac<F>* __restrict__ tails, uint32_t* __restrict__ big,
+                                                  uint32_t* __restrict__ nbig) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   const uint32_t bs = offsets[b], be = offsets[b + 1];
@@ -255,9 +262,37 @@ __global__ void __launch_bounds__(64) k_msm_fixup(const uint32_t* __restrict__ o
   }
   const uint32_t cs = bs / MSM_CHUNK, ce = (be - 1) / MSM_CHUNK;
   if (cs == ce) return;
+  if (ce - cs > FIX_SERIAL) {
+    big[atomicAdd(nbig, 1u)] = b;
+    return;
+  }
   jac<F> acc = tails[cs];
   for (uint32_t c = cs + 1; c <= ce; ++c) acc = jac_add(acc, heads[c]);
   buckets[b] = acc;
+}
+
+// One 64-thread workgroup per queued bucket: strided partial sums, then a tree in LDS.  The grid
+// is sized for the most buckets that can span more than FIX_SERIAL chunks; surplus groups exit.
+template <class F>
+__global__ void __launch_bounds__(64) k_msm_fixup_big(const uint32_t* __restrict__ offsets,
+                                                      jac<F>* __restrict__ buckets, const jac<F>* __restrict__ heads,
+                                                      const jac<F>* __restrict__ tails, const uint32_t* __restrict__ big,
+                                                      const uint32_t* __restrict__ nbig) {
+  __shared__ jac<F> sh[64];
+  if (blockIdx.x >= *nbig) return;
+  const uint32_t b = big[blockIdx.x];
+  const uint32_t bs = offsets[b], be = offsets[b + 1];
+  const uint32_t cs = bs / MSM_CHUNK, ce = (be - 1) / MSM_CHUNK;
+  const int tid = threadIdx.x;
+  jac<F> acc = tid == 0 ? tails[cs] : jac_identity<F>();
+  for (uint32_t c = cs + 1 + tid; c <= ce; c += 64) acc = jac_add(acc, heads[c]);
+  sh[tid] = acc;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (tid < w) sh[tid] = jac_add(sh[tid], sh[tid + w]);
+    __syncthreads();
+  }
+  if (tid == 0) buckets[b] = sh[0];
 }
 
 template <class F>
@@ -304,24 +339,50 @@ __global__ void __launch_bounds__(T) k_msm_window_sums(const jac<F>* __restrict_
 // ---------------------------------------------------------------------------------------------
 // Host driver
 // ---------------------------------------------------------------------------------------------
-template <class T> static int dalloc(T** p, size_t count, hipStream_t s) {
-  if (count == 0) count = 1;
-  hipError_t e = hipMallocAsync((void**)p, count * sizeof(T), s);
-  if (e != hipSuccess) return set_error(BTE_ENOMEM, "hipMallocAsync(%zu): %s", count * sizeof(T),
-                                        hipGetErrorString(e));
-  return BTE_OK;
+// Grow-only device workspace per (device, stream).  An MSM call holds its entry's lock from
+// the first launch to its final stream synchronisation, so calls on one stream serialise and
+// calls on different streams use different memory.
+struct Workspace {
+  std::mutex mu;
+  void* base = nullptr;
+  size_t cap = 0;
+};
+static std::mutex g_ws_mu;
+static std::map<std::pair<int, hipStream_t>, Workspace*> g_ws;
+
+static Workspace* workspace_for(hipStream_t s) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Workspace*& w = g_ws[std::make_pair(dev, s)];
+  if (!w) w = new Workspace();
+  return w;
 }
 
-struct AsyncFreeList {
-  hipStream_t s;
-  std::vector<void*> ptrs;
-  ~AsyncFreeList() {
-    for (void* p : ptrs) (void)hipFreeAsync(p, s);
+// Carves typed, 256-byte aligned sub-buffers out of one allocation.
+struct Carver {
+  size_t off = 0;
+  std::vector<std::pair<void**, size_t>> items;
+  template <class T> void add(T** p, size_t count) {
+    items.push_back({(void**)p, (count ? count : 1) * sizeof(T)});
   }
-  template <class T> int alloc(T** p, size_t count) {
-    int rc = dalloc(p, count, s);
-    if (rc == BTE_OK) ptrs.push_back((void*)*p);
-    return rc;
+  int commit(Workspace* ws) {
+    size_t total = 0;
+    for (auto& it : items) total += (it.second + 255) & ~(size_t)255;
+    if (total > ws->cap) {
+      if (ws->base) BTE_HIP(hipFree(ws->base));
+      ws->base = nullptr;
+      ws->cap = 0;
+      hipError_t e = hipMalloc(&ws->base, total);
+      if (e != hipSuccess) return set_error(BTE_ENOMEM, "msm workspace (%zu bytes): %s", total, hipGetErrorString(e));
+      ws->cap = total;
+    }
+    char* b = (char*)ws->base;
+    for (auto& it : items) {
+      *it.first = b;
+      b += (it.second + 255) & ~(size_t)255;
+    }
+    return BTE_OK;
   }
 };
 
@@ -347,26 +408,39 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
   const int ntiles = (int)((nb + 1023) / 1024);
   if (ntiles > (1 << 20)) return set_error(BTE_EINVAL, "msm: window too wide");
 
-  AsyncFreeList fl{s, {}};
+  Workspace* ws = workspace_for(s);
+  std::lock_guard<std::mutex> ws_lock(ws->mu);
+  Carver cv;
   int32_t* digits;
   uint32_t *counts, *offsets, *cursor, *sorted, *tile_sums;
   jac<F> *buckets, *heads, *tails, *seg_acc, *seg_run, *wsum;
   int* err;
+  uint32_t *big, *nbig;
+  const size_t max_big = nchunks / FIX_SERIAL + 1;
   int rc;
-  if ((rc = fl.alloc(&digits, E_max)) || (rc = fl.alloc(&counts, nb)) ||
-      (rc = fl.alloc(&offsets, (size_t)nb + 1)) || (rc = fl.alloc(&cursor, nb)) ||
-      (rc = fl.alloc(&sorted, E_max)) || (rc = fl.alloc(&tile_sums, ntiles)) ||
-      (rc = fl.alloc(&buckets, nb)) || (rc = fl.alloc(&heads, nchunks)) ||
-      (rc = fl.alloc(&tails, nchunks)) || (rc = fl.alloc(&seg_acc, sh.nw * S)) ||
-      (rc = fl.alloc(&seg_run, sh.nw * S)) || (rc = fl.alloc(&wsum, (size_t)sh.nw * nsum)) ||
-      (rc = fl.alloc(&err, 1)))
-    return rc;
+  cv.add(&digits, E_max);
+  cv.add(&counts, nb);
+  cv.add(&offsets, (size_t)nb + 1);
+  cv.add(&cursor, nb);
+  cv.add(&sorted, E_max);
+  cv.add(&tile_sums, ntiles);
+  cv.add(&buckets, nb);
+  cv.add(&heads, nchunks);
+  cv.add(&tails, nchunks);
+  cv.add(&seg_acc, sh.nw * S);
+  cv.add(&seg_run, sh.nw * S);
+  cv.add(&wsum, (size_t)sh.nw * nsum);
+  cv.add(&err, 1);
+  cv.add(&big, max_big);
+  cv.add(&nbig, 1);
+  if ((rc = cv.commit(ws))) return rc;
 
   PhaseTimer pt(s);
   pt.begin();
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, counts, (size_t)nb, 0u);
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, cursor, (size_t)nb, 0u);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)err, (size_t)1, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nbig, (size_t)1, 0u);
   hipLaunchKernelGGL(k_msm_digits, dim3(blocks_for(n, 256)), dim3(256), 0, s,
                      (const uint32_t*)scalars, n, sh.c, sh.nw, sh.M, digits, counts);
   pt.mark("digits");
@@ -382,7 +456,11 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
                      buckets, heads, tails, err);
   pt.mark("accumulate");
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup<F>), dim3(blocks_for(nb, 64)), dim3(64), 0, s,
-                     (const uint32_t*)offsets, nb, buckets, (const jac<F>*)heads, (const jac<F>*)tails);
+                     (const uint32_t*)offsets, nb, buckets, (const jac<F>*)heads, (const jac<F>*)tails, big,
+                     nbig);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_big<F>), dim3((unsigned)max_big), dim3(64), 0, s,
+                     (const uint32_t*)offsets, buckets, (const jac<F>*)heads, (const jac<F>*)tails,
+                     (const uint32_t*)big, (const uint32_t*)nbig);
   pt.mark("fixup");
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_segsum<F>), dim3(blocks_for(sh.nw * S, 64)), dim3(64), 0, s,
                      (const jac<F>*)buckets, sh.nw, sh.M, sh.logL, sh.logS, seg_acc, seg_run);

@@ -1,0 +1,276 @@
+// Optimal-ate Miller loops and the split final exponentiation of BLS12-381 on gfx950.
+//
+// Replaces, from weighted_btx/src/final_exponentiation.rs:
+//   PreparedG2Lines::from_affine        :186-195  (blst_precompute_lines)   -> bte_g2_prepare_lines
+//   PreparedG2Lines::miller_loop        :207-219  (blst_miller_loop_lines)  -> bte_miller_loop_prepared
+//   unprepared_multi_miller_loop        :229-273  (blst_miller_loop_n)      -> bte_multi_miller_loop
+//   easy / batch_easy / hard / full     :276-354                            -> bte_final_exp_*
+//
+// Miller loop: f = prod over the bits of |x| (x = -0xd201000000010000) of doubling lines, with
+// addition lines at the 5 set bits below the top; 63 + 5 = 68 lines per G2 point, then f is
+// conjugated because x < 0.  T runs in homogeneous projective coordinates on the twist
+// y^2 = x^3 + 4(u+1); a line through T evaluated at P = (xp, yp) in G1, scaled by a factor that
+// the final exponentiation removes, is
+//     l = l0 + (cx * xp) v + (cy * yp) v w     (three Fp2 coefficients, an Fp6-sized record)
+//   doubling:  l0 = Y^2 - 3b'Z^2, cx = -3X^2, cy = 2YZ
+//   addition:  theta = Y - yq Z, lambda = X - xq Z; l0 = theta xq - lambda yq, cx = -theta,
+//              cy = lambda
+// Raw Miller outputs therefore differ from blst's by such factors; values agree after the final
+// exponentiation, which is where the reference's outputs are canonical.
+//
+// A multi-Miller loop over a segment of pairs equals the product of the per-pair loops (the
+// shared squaring distributes over the product), so pairs run one per thread and each segment
+// is multiplied together by one workgroup.
+#include "common.hpp"
+#include "curve.hpp"
+#include "cyclo.hpp"
+
+namespace bte {
+
+static constexpr uint64_t MILLER_X_ABS = 0xd201000000010000ull;
+static constexpr int MILLER_LINES = 68;
+
+struct line_t {
+  fp2 l0, cx, cy;  // laid out as one blst_fp6
+};
+static_assert(sizeof(line_t) == 288, "line layout");
+
+// T <- 2T and its tangent line.
+BTE_DEV line_t dbl_step(g2j& T) {
+  const fp2 X = T.x, Y = T.y, Z = T.z;
+  fp2 A = fp2_half(fp2_mul(X, Y));
+  fp2 B = fp2_sqr(Y);
+  fp2 C = fp2_sqr(Z);
+  fp2 C3 = fp2_mul3(fp2_mul_xi(C));           // 3 (u+1) Z^2
+  fp2 E = fp2_dbl(fp2_dbl(C3));                // 3 b' Z^2 with b' = 4 (u+1)
+  fp2 F = fp2_mul3(E);
+  fp2 G = fp2_half(fp2_add(B, F));
+  fp2 H = fp2_sub(fp2_sqr(fp2_add(Y, Z)), fp2_add(B, C));
+  line_t l;
+  l.l0 = fp2_sub(B, E);
+  l.cx = fp2_neg(fp2_mul3(fp2_sqr(X)));
+  l.cy = H;
+  T.x = fp2_mul(A, fp2_sub(B, F));
+  T.y = fp2_sub(fp2_sqr(G), fp2_mul3(fp2_sqr(E)));
+  T.z = fp2_mul(B, H);
+  return l;
+}
+
+// T <- T + Q (Q affine) and the chord line.
+BTE_DEV line_t add_step(g2j& T, const g2a& Q) {
+  fp2 theta = fp2_sub(T.y, fp2_mul(Q.y, T.z));
+  fp2 lambda = fp2_sub(T.x, fp2_mul(Q.x, T.z));
+  fp2 C = fp2_sqr(theta);
+  fp2 D = fp2_sqr(lambda);
+  fp2 E = fp2_mul(lambda, D);
+  fp2 F = fp2_mul(T.z, C);
+  fp2 G = fp2_mul(T.x, D);
+  fp2 H = fp2_sub(fp2_add(E, F), fp2_dbl(G));
+  line_t l;
+  l.l0 = fp2_sub(fp2_mul(theta, Q.x), fp2_mul(lambda, Q.y));
+  l.cx = fp2_neg(theta);
+  l.cy = lambda;
+  T.x = fp2_mul(lambda, H);
+  T.y = fp2_sub(fp2_mul(theta, fp2_sub(G, H)), fp2_mul(T.y, E));
+  T.z = fp2_mul(T.z, E);
+  return l;
+}
+
+BTE_DEV fp12 apply_line(const fp12& f, const line_t& l, const g1a& P) {
+  return fp12_mul_line(f, l.l0, fp2_mul_fp(l.cx, P.x), fp2_mul_fp(l.cy, P.y));
+}
+
+// One Miller loop f_{|x|,Q}(P), conjugated; identity inputs give one.
+BTE_DEV fp12 miller_loop_unprepared(const g1a& P, const g2a& Q) {
+  if (aff_is_identity(P) || aff_is_identity(Q)) return fp12_one();
+  g2j T = {Q.x, Q.y, fp2_one()};
+  fp12 f = fp12_one();
+#pragma unroll 1
+  for (int b = 62; b >= 0; --b) {
+    if (b != 62) f = fp12_sqr(f);
+    f = apply_line(f, dbl_step(T), P);
+    if ((MILLER_X_ABS >> b) & 1) f = apply_line(f, add_step(T, Q), P);
+  }
+  return fp12_conj(f);
+}
+
+__global__ void __launch_bounds__(64) k_prepare_lines(const g2a* __restrict__ q, size_t n, line_t* __restrict__ lines) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  line_t* out = lines + i * MILLER_LINES;
+  const g2a Q = q[i];
+  if (aff_is_identity(Q)) {
+    // an all-zero table marks the identity (a real first line has cy = 2YZ != 0)
+    for (int k = 0; k < MILLER_LINES; ++k) out[k] = {fp2_zero(), fp2_zero(), fp2_zero()};
+    return;
+  }
+  g2j T = {Q.x, Q.y, fp2_one()};
+  int k = 0;
+#pragma unroll 1
+  for (int b = 62; b >= 0; --b) {
+    out[k++] = dbl_step(T);
+    if ((MILLER_X_ABS >> b) & 1) out[k++] = add_step(T, Q);
+  }
+}
+
+__global__ void __launch_bounds__(64) k_miller_prepared(const line_t* __restrict__ lines, const g1a* __restrict__ p,
+                                                        size_t n, fp12* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const line_t* L = lines + i * MILLER_LINES;
+  const g1a P = p[i];
+  if (aff_is_identity(P) || fp2_is_zero(L[0].cy)) {
+    out[i] = fp12_one();
+    return;
+  }
+  fp12 f = fp12_one();
+  int k = 0;
+#pragma unroll 1
+  for (int b = 62; b >= 0; --b) {
+    if (b != 62) f = fp12_sqr(f);
+    f = apply_line(f, L[k++], P);
+    if ((MILLER_X_ABS >> b) & 1) f = apply_line(f, L[k++], P);
+  }
+  out[i] = fp12_conj(f);
+}
+
+__global__ void __launch_bounds__(64) k_miller_pairs(const g1a* __restrict__ p, const g2a* __restrict__ q, size_t n,
+                                                     fp12* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = miller_loop_unprepared(p[i], q[i]);
+}
+
+// out[s] = prod of f[off[s] .. off[s+1]); one 64-thread workgroup per segment.
+__global__ void __launch_bounds__(64) k_segment_product(const fp12* __restrict__ f, const uint64_t* __restrict__ off,
+                                                        size_t nseg, fp12* __restrict__ out) {
+  __shared__ fp12 sh[64];
+  const size_t s = blockIdx.x;
+  if (s >= nseg) return;
+  const uint64_t a = off[s], b = off[s + 1];
+  const int tid = threadIdx.x;
+  fp12 acc = fp12_one();
+  bool any = false;
+  for (uint64_t i = a + tid; i < b; i += 64) {
+    acc = any ? fp12_mul(acc, f[i]) : f[i];
+    any = true;
+  }
+  sh[tid] = acc;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (tid < w && a + tid + w < b) sh[tid] = fp12_mul(sh[tid], sh[tid + w]);
+    __syncthreads();
+  }
+  if (tid == 0) out[s] = sh[0];
+}
+
+// Easy part over chunks of EASY_CHUNK elements sharing one Fp12 inversion (Montgomery's trick,
+// as batch_easy_final_exponentiation does for the whole batch; the values are identical).
+static constexpr int EASY_CHUNK = 4;
+__global__ void __launch_bounds__(64) k_final_exp_easy(const fp12* __restrict__ in, size_t n, fp12* __restrict__ out,
+                                                       int also_hard) {
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t start = t * EASY_CHUNK;
+  if (start >= n) return;
+  const size_t stop = start + EASY_CHUNK < n ? start + EASY_CHUNK : n;
+  // prefix products staged in out[]
+  fp12 acc = fp12_one();
+  for (size_t i = start; i < stop; ++i) {
+    out[i] = acc;
+    acc = fp12_mul(acc, in[i]);
+  }
+  fp12 inv = fp12_inv(acc);
+  for (size_t i = stop; i-- > start;) {
+    const fp12 f = in[i];
+    const fp12 fi = fp12_mul(inv, out[i]);
+    inv = fp12_mul(inv, f);
+    fp12 e = easy_part_with_inverse(f, fi);
+    out[i] = also_hard ? hard_part(e) : e;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_final_exp_hard(const fp12* __restrict__ in, size_t n, fp12* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = hard_part(in[i]);
+}
+
+}  // namespace bte
+
+using namespace bte;
+
+extern "C" int bte_g2_prepare_lines(const bte_p2_affine* q, size_t n, bte_fp6* lines, void* stream) {
+  if (n == 0) return BTE_OK;
+  if (!q || !lines) return set_error(BTE_EINVAL, "prepare_lines: null pointer");
+  hipLaunchKernelGGL(k_prepare_lines, dim3(blocks_for(n, 64)), dim3(64), 0, (hipStream_t)stream,
+                     (const g2a*)q, n, (line_t*)lines);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+extern "C" int bte_miller_loop_prepared(const bte_fp6* lines, const bte_p1_affine* p, size_t n, bte_fp12* out,
+                                        void* stream) {
+  if (n == 0) return BTE_OK;
+  if (!lines || !p || !out) return set_error(BTE_EINVAL, "miller_loop_prepared: null pointer");
+  hipLaunchKernelGGL(k_miller_prepared, dim3(blocks_for(n, 64)), dim3(64), 0, (hipStream_t)stream,
+                     (const line_t*)lines, (const g1a*)p, n, (fp12*)out);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+extern "C" int bte_miller_loop_pairs(const bte_p1_affine* p, const bte_p2_affine* q, size_t n, bte_fp12* out,
+                                     void* stream) {
+  if (n == 0) return BTE_OK;
+  if (!p || !q || !out) return set_error(BTE_EINVAL, "miller_loop_pairs: null pointer");
+  hipLaunchKernelGGL(k_miller_pairs, dim3(blocks_for(n, 64)), dim3(64), 0, (hipStream_t)stream,
+                     (const g1a*)p, (const g2a*)q, n, (fp12*)out);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+extern "C" int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine* q, size_t npairs,
+                                     const uint64_t* seg_offsets, size_t nseg, bte_fp12* out, void* stream) {
+  if (nseg == 0) return BTE_OK;
+  if (!seg_offsets || !out || (npairs && (!p || !q)))
+    return set_error(BTE_EINVAL, "multi_miller_loop: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  fp12* f = nullptr;
+  BTE_HIP(hipMallocAsync((void**)&f, (npairs ? npairs : 1) * sizeof(fp12), s));
+  if (npairs)
+    hipLaunchKernelGGL(k_miller_pairs, dim3(blocks_for(npairs, 64)), dim3(64), 0, s, (const g1a*)p,
+                       (const g2a*)q, npairs, f);
+  hipLaunchKernelGGL(k_segment_product, dim3((unsigned)nseg), dim3(64), 0, s, (const fp12*)f, seg_offsets, nseg,
+                     (fp12*)out);
+  (void)hipFreeAsync(f, s);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+extern "C" int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, void* stream) {
+  if (n == 0) return BTE_OK;
+  if (!in || !out || (const void*)in == (const void*)out)
+    return set_error(BTE_EINVAL, "final_exp_easy: null or aliased buffers");
+  hipLaunchKernelGGL(k_final_exp_easy, dim3(blocks_for((n + EASY_CHUNK - 1) / EASY_CHUNK, 64)), dim3(64), 0,
+                     (hipStream_t)stream, (const fp12*)in, n, (fp12*)out, 0);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+extern "C" int bte_final_exp_hard(const bte_fp12* in, size_t n, bte_fp12* out, void* stream) {
+  if (n == 0) return BTE_OK;
+  if (!in || !out) return set_error(BTE_EINVAL, "final_exp_hard: null pointer");
+  hipLaunchKernelGGL(k_final_exp_hard, dim3(blocks_for(n, 64)), dim3(64), 0, (hipStream_t)stream,
+                     (const fp12*)in, n, (fp12*)out);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+extern "C" int bte_final_exp(const bte_fp12* in, size_t n, bte_fp12* out, void* stream) {
+  if (n == 0) return BTE_OK;
+  if (!in || !out || (const void*)in == (const void*)out)
+    return set_error(BTE_EINVAL, "final_exp: null or aliased buffers");
+  hipLaunchKernelGGL(k_final_exp_easy, dim3(blocks_for((n + EASY_CHUNK - 1) / EASY_CHUNK, 64)), dim3(64), 0,
+                     (hipStream_t)stream, (const fp12*)in, n, (fp12*)out, 1);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}

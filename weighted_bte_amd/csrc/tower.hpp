@@ -9,6 +9,11 @@
 
 namespace bte {
 
+// Out-of-line device functions: the Fp2 products and every Fp12-level operation are real
+// calls, so kernels built from them (Miller loops, final exponentiation, GT transforms) stay
+// small enough for the instruction cache and for the compiler.
+#define BTE_NOINL static __device__ __attribute__((noinline))
+
 struct fp2 {
   fp c0, c1;
 };
@@ -34,7 +39,7 @@ BTE_DEV fp2 fp2_conj(const fp2& a) { return {a.c0, fp_neg(a.c1)}; }
 BTE_DEV fp2 fp2_half(const fp2& a) { return {fp_half(a.c0), fp_half(a.c1)}; }
 
 // Karatsuba: 3 Fp multiplications.
-BTE_DEV fp2 fp2_mul(const fp2& a, const fp2& b) {
+BTE_NOINL fp2 fp2_mul(const fp2& a, const fp2& b) {
   fp t0 = fp_mul(a.c0, b.c0);
   fp t1 = fp_mul(a.c1, b.c1);
   fp t2 = fp_mul(fp_add(a.c0, a.c1), fp_add(b.c0, b.c1));
@@ -42,7 +47,7 @@ BTE_DEV fp2 fp2_mul(const fp2& a, const fp2& b) {
 }
 
 // (a0 + a1 u)^2 = (a0 + a1)(a0 - a1) + 2 a0 a1 u: 2 Fp multiplications.
-BTE_DEV fp2 fp2_sqr(const fp2& a) {
+BTE_NOINL fp2 fp2_sqr(const fp2& a) {
   fp t = fp_mul(a.c0, a.c1);
   return {fp_mul(fp_add(a.c0, a.c1), fp_sub(a.c0, a.c1)), fp_dbl(t)};
 }
@@ -54,7 +59,7 @@ BTE_DEV fp2 fp2_mul_xi(const fp2& a) { return {fp_sub(a.c0, a.c1), fp_add(a.c0, 
 
 BTE_DEV fp2 fp2_mul3(const fp2& a) { return fp2_add(fp2_dbl(a), a); }
 
-BTE_DEV fp2 fp2_inv(const fp2& a) {
+BTE_NOINL fp2 fp2_inv(const fp2& a) {
   fp n = fp_add(fp_sqr(a.c0), fp_sqr(a.c1));
   fp ni = fp_inv(n);
   return {fp_mul(a.c0, ni), fp_neg(fp_mul(a.c1, ni))};
@@ -110,7 +115,7 @@ BTE_DEV fp6 fp6_mul_1(const fp6& a, const fp2& b1) {
   return {fp2_mul_xi(fp2_mul(a.c2, b1)), fp2_mul(a.c0, b1), fp2_mul(a.c1, b1)};
 }
 
-BTE_DEV fp6 fp6_inv(const fp6& a) {
+BTE_NOINL fp6 fp6_inv(const fp6& a) {
   fp2 t0 = fp2_sub(fp2_sqr(a.c0), fp2_mul_xi(fp2_mul(a.c1, a.c2)));
   fp2 t1 = fp2_sub(fp2_mul_xi(fp2_sqr(a.c2)), fp2_mul(a.c0, a.c1));
   fp2 t2 = fp2_sub(fp2_sqr(a.c1), fp2_mul(a.c0, a.c2));
@@ -128,7 +133,7 @@ BTE_DEV bool fp12_eq(const fp12& a, const fp12& b) { return fp6_eq(a.c0, b.c0) &
 BTE_DEV fp12 fp12_conj(const fp12& a) { return {a.c0, fp6_neg(a.c1)}; }
 
 // Karatsuba over Fp6: 18 Fp2 multiplications.
-BTE_DEV fp12 fp12_mul(const fp12& a, const fp12& b) {
+BTE_NOINL fp12 fp12_mul(const fp12& a, const fp12& b) {
   fp6 t0 = fp6_mul(a.c0, b.c0);
   fp6 t1 = fp6_mul(a.c1, b.c1);
   fp6 c1 = fp6_sub(fp6_mul(fp6_add(a.c0, a.c1), fp6_add(b.c0, b.c1)), fp6_add(t0, t1));
@@ -136,7 +141,7 @@ BTE_DEV fp12 fp12_mul(const fp12& a, const fp12& b) {
 }
 
 // Complex squaring: 2 Fp6 multiplications.
-BTE_DEV fp12 fp12_sqr(const fp12& a) {
+BTE_NOINL fp12 fp12_sqr(const fp12& a) {
   fp6 t = fp6_mul(a.c0, a.c1);
   fp6 c0 = fp6_sub(fp6_mul(fp6_add(a.c0, a.c1), fp6_add(a.c0, fp6_mul_v(a.c1))),
                    fp6_add(t, fp6_mul_v(t)));
@@ -144,14 +149,14 @@ BTE_DEV fp12 fp12_sqr(const fp12& a) {
 }
 
 // f * l for a Miller-loop line l = (l0 + l1 v) + (l4 v) w.
-BTE_DEV fp12 fp12_mul_line(const fp12& f, const fp2& l0, const fp2& l1, const fp2& l4) {
+BTE_NOINL fp12 fp12_mul_line(const fp12& f, const fp2& l0, const fp2& l1, const fp2& l4) {
   fp6 a = fp6_mul_01(f.c0, l0, l1);
   fp6 b = fp6_mul_1(f.c1, l4);
   fp6 c = fp6_mul_01(fp6_add(f.c0, f.c1), l0, fp2_add(l1, l4));
   return {fp6_add(a, fp6_mul_v(b)), fp6_sub(c, fp6_add(a, b))};
 }
 
-BTE_DEV fp12 fp12_inv(const fp12& a) {
+BTE_NOINL fp12 fp12_inv(const fp12& a) {
   fp6 den = fp6_sub(fp6_mul(a.c0, a.c0), fp6_mul_v(fp6_mul(a.c1, a.c1)));
   fp6 di = fp6_inv(den);
   return {fp6_mul(a.c0, di), fp6_neg(fp6_mul(a.c1, di))};
@@ -169,7 +174,7 @@ BTE_DEV fp2 frob_gamma(int k, int i) {
 
 // x -> x^(p^k), k in 1..3.  In the w basis x = sum g_i w^i with (g0, g2, g4) = c0 and
 // (g1, g3, g5) = c1; each coefficient is conjugated for odd k and scaled by gamma_{k,i}.
-BTE_DEV fp12 fp12_frobenius(const fp12& a, int k) {
+BTE_NOINL fp12 fp12_frobenius(const fp12& a, int k) {
   const bool odd = k & 1;
   auto f = [&](const fp2& g, int i) {
     fp2 x = odd ? fp2_conj(g) : g;
@@ -195,7 +200,7 @@ BTE_DEV void fp4_sqr(fp2& c0, fp2& c1, const fp2& a, const fp2& b) {
 
 // Granger-Scott squaring, valid on the cyclotomic subgroup.  With f = A + B w + C w^2 over
 // Fp4 = Fp2[w^3]: f^2 = (3A^2 - 2conj A) + (3 s C^2 + 2 conj B) w + (3B^2 - 2 conj C) w^2.
-BTE_DEV fp12 fp12_cyclotomic_sqr(const fp12& f) {
+BTE_NOINL fp12 fp12_cyclotomic_sqr(const fp12& f) {
   fp2 z0 = f.c0.c0, z4 = f.c0.c1, z3 = f.c0.c2, z2 = f.c1.c0, z1 = f.c1.c1, z5 = f.c1.c2;
   fp2 t0, t1, t2, t3;
   fp4_sqr(t0, t1, z0, z1);

@@ -49,14 +49,19 @@ class Stream:
         self.key = hashlib.sha256(b"weighted-bte-amd/%d/%d" % (seed, stream)).digest()
         self.counter = 0
         self.buf = b""
+        self.pos = 0
 
     def bytes(self, n: int) -> bytes:
-        if len(self.buf) < n:
-            need = n - len(self.buf)
-            blocks = (need + 63) // 64
-            self.buf += chacha20_blocks(self.key, self.counter, blocks)
+        if len(self.buf) - self.pos < n:
+            rest = self.buf[self.pos:]
+            need = n - len(rest)
+            # refill ahead in bulk: the byte sequence is the same, small reads stay cheap
+            blocks = max((need + 63) // 64, 1024)
+            self.buf = rest + chacha20_blocks(self.key, self.counter, blocks)
+            self.pos = 0
             self.counter += blocks
-        out, self.buf = self.buf[:n], self.buf[n:]
+        out = self.buf[self.pos:self.pos + n]
+        self.pos += n
         return out
 
     def scalars(self, n: int):
