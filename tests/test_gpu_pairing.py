@@ -98,8 +98,8 @@ def test_multi_miller_uneven_segments(api):
 def test_bilinearity_on_gpu(api):
     rnd = random.Random(5)
     a, b = rnd.randrange(1, O.R), rnd.randrange(1, O.R)
-    P = [O.G1_GEN, g1_mul_gen(a)]
-    Q = [g2_mul_gen(b), O.G2_GEN]
+    P = [g1_mul_gen(a), O.G1_GEN]
+    Q = [g2_mul_gen(b), g2_mul_gen(a * b)]
     e = read_f12(api, api.full_final_exponentiation(
         api.miller_loop_pairs(api.to_device(g1_points_bytes(P)), api.to_device(g2_points_bytes(Q)))), 2)
     assert e[0] == e[1]
