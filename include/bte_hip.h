@@ -63,6 +63,19 @@ int bte_g2_msm(const bte_p2_affine* points, const uint8_t* scalars, size_t n, bt
 int bte_g2_msm_indexed(const bte_p2_affine* points, size_t n_points, const uint32_t* indices,
                        const uint8_t* scalars, size_t n, bte_p2* out, void* stream);
 
+/* Batched MSMs of uneven sizes in one asynchronous launch sequence (no host synchronisation,
+ * graph-capturable): out[s] = sum over terms t in [seg_offsets[s], seg_offsets[s+1]) of
+ * scalars[t] * P(t), P(t) = points[indices[t]] when `indices` is non-NULL (bounds-checked against
+ * n_points) else points[t].  seg_offsets is a device array of nseg+1 u64 with
+ * seg_offsets[nseg] = nterms.  Replaces the rayon fan-out of per-party / per-slot MSMs around
+ * weighted_btx/src/decryption.rs:205-233 (partial_decrypt) and :465-481 (prepare_decryption). */
+int bte_g1_msm_batched(const bte_p1_affine* points, size_t n_points, const uint32_t* indices,
+                       const uint8_t* scalars, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
+                       bte_p1* out, void* stream);
+int bte_g2_msm_batched(const bte_p2_affine* points, size_t n_points, const uint32_t* indices,
+                       const uint8_t* scalars, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
+                       bte_p2* out, void* stream);
+
 /* Window width override for the Pippenger kernels (0 = the built-in table by size). */
 int bte_msm_set_window(int g1_or_g2, size_t n, int window_bits);
 
@@ -125,6 +138,41 @@ int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, void* stream
 int bte_final_exp_hard(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
 /* final_exponentiation.rs:348-354 full_final_exponentiation (in and out must not alias) */
 int bte_final_exp(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
+
+/* ---------------------------------------------------------------------------------------- *
+ * a16: weighted_btx batch-decryption phases (host control flow over the kernels above).
+ * Key material (device) is one G2 affine array: the reference's negative_material (L blocks of
+ * W points) followed by its positive_material (L-1 blocks of W).  Host arrays are marked.
+ * ---------------------------------------------------------------------------------------- */
+
+/* weighted_btx/src/interpolation.rs:31-52 lagrange_at_zero over the first W points of the
+ * radix-2 domain of size nextpow2(W); selected (n indices) and out (n x 32 bytes) are HOST. */
+int bte_lagrange_at_zero(size_t total_weight, const uint64_t* selected, size_t n, uint8_t* out);
+
+/* weighted_btx/src/decryption.rs:427-551 prepare_decryption.  party_offsets (N+1) and accepted
+ * (T party indices, committee order) are HOST.  Outputs (device): kernel_lines, nextpow2(2B)
+ * tables of 68 lines; opening_keys, B x T G2 affine (slot-major). */
+int bte_btx_prepare(const bte_p2_affine* material, size_t total_weight, size_t max_batch,
+                    const uint64_t* party_offsets, size_t party_count, const uint32_t* accepted,
+                    size_t n_accepted, size_t batch_size, bte_fp6* kernel_lines,
+                    bte_p2_affine* opening_keys, void* stream);
+
+/* decryption.rs:555-603 precompute_batch: ct1 = B G1 affine (identity in invalid slots),
+ * valid = B HOST flags; beta_out = B Fp12 (device). */
+int bte_btx_precompute(const bte_p1_affine* ct1, const uint8_t* valid, size_t batch_size,
+                       const bte_fp6* kernel_lines, bte_fp12* beta_out, void* stream);
+
+/* decryption.rs:607-695 open_batch: shares = T G1 Jacobian (committee order), ct2 and beta =
+ * B Fp12, valid = B HOST flags; messages_out = B Fp12 (one in invalid slots, the reference's
+ * None). */
+int bte_btx_open(const bte_p1* shares, size_t n_accepted, const bte_p2_affine* opening_keys,
+                 const bte_fp12* ct2, const bte_fp12* beta, const uint8_t* valid, size_t batch_size,
+                 bte_fp12* messages_out, void* stream);
+
+/* decryption.rs:205-233 partial_decrypt for N parties at once: sigma_j = sum_i q_j^i ct1_i,
+ * i = 1..B.  q = N canonical scalars (HOST); sigma_out = N G1 Jacobian (device). */
+int bte_btx_partial_decrypt(const bte_p1_affine* ct1, size_t batch_size, const uint8_t* q,
+                            size_t n_parties, bte_p1* sigma_out, void* stream);
 
 /* ---------------------------------------------------------------------------------------- *
  * Fixed-base generation of test and key material: out[i] = k_i * G for the standard

@@ -1,0 +1,89 @@
+"""End-to-end weighted_btx batch decryption on the GPU (weighted_btx/src/decryption.rs):
+decrypt(encrypt(m)) == m bit-exactly, and beta equals the reference's literal weighted
+middle-product formula (decryption.rs:806-842) in closed form."""
+import random
+
+import pytest
+
+from btx_setup import BtxSetup, heaviest_first_committee
+from helpers import O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api(dev):
+    from weighted_bte_amd import api as A
+    return A
+
+
+def read_f12(api, t, n):
+    b = api.to_host(t)
+    return [O.f12_from_bytes(b[576 * i: 576 * i + 576]) for i in range(n)]
+
+
+def run_btx(api, st, B, invalid=()):
+    ct1, ct2 = st.encrypt_batch(B)
+    valid = [i not in invalid for i in range(B)]
+    if invalid:
+        raw = bytearray(api.to_host(ct1))
+        for i in invalid:
+            raw[96 * i: 96 * i + 96] = bytes(96)
+        ct1 = api.to_device(bytes(raw))
+    committee = heaviest_first_committee(st.weights, st.t)
+    sigma_all = api.btx_partial_decrypt(ct1, st.q)
+    shares = b"".join(api.to_host(sigma_all[144 * j: 144 * j + 144]) for j in committee)
+    lines, keys = api.btx_prepare(st.material, st.W, st.L, st.offsets, committee, B)
+    beta = api.btx_precompute(ct1, valid, lines)
+    msgs = api.btx_open(api.to_device(shares), keys, ct2, beta, valid)
+    return committee, valid, beta, msgs
+
+
+def test_lagrange_matches_oracle_definition(api):
+    rnd = random.Random(3)
+    W = 45
+    dom = O.Radix2Domain(64)
+    for sel in ([0], [3, 1, 7], sorted(rnd.sample(range(W), 20)), list(range(W))):
+        got = api.lagrange_at_zero(W, sel)
+        xs = [pow(dom.generator, v, O.R) for v in sel]
+        for i, xi in enumerate(xs):
+            num, den = 1, 1
+            for j, xj in enumerate(xs):
+                if j != i:
+                    num = num * xj % O.R
+                    den = den * (xj - xi) % O.R
+            assert got[i] == num * pow(den, -1, O.R) % O.R
+    # complete power-of-two domain: every coefficient is 1/M (interpolation.rs:325-338)
+    assert api.lagrange_at_zero(16, list(range(16))) == [pow(16, -1, O.R)] * 16
+
+
+@pytest.mark.parametrize("B", [1, 2, 3, 7, 8])
+def test_btx_beta_matches_literal_formula_and_decrypts(api, B):
+    st = BtxSetup(api, [1, 2, 3], threshold=3, max_batch=8, seed=B)
+    committee, valid, beta, msgs = run_btx(api, st, B)
+    sel = [v for j in committee for v in range(st.offsets[j], st.offsets[j + 1])]
+    lam = api.lagrange_at_zero(st.W, sel)
+    g = O.gt_generator()
+    got_beta = read_f12(api, beta, B)
+    for i in range(B):
+        e = sum(st.r[k] * st.kappa(sel, lam, k - i) for k in range(B) if k != i) % O.R
+        assert got_beta[i] == O.f12_pow(g, e), "beta slot %d" % i
+    got = read_f12(api, msgs, B)
+    assert got == [O.f12_pow(g, s) for s in st.s]
+
+
+def test_btx_config1_shape_with_invalid_slot(api):
+    """n = 8 parties, weights U[1,8] from the seed, threshold = W // 2, B = L = 32; slot 5 is
+    invalid (identity first component) and opens to None (one)."""
+    rng = O.SeededRng(11, 1)
+    weights = [1 + rng.below(8) for _ in range(8)]
+    W = sum(weights)
+    st = BtxSetup(api, weights, threshold=W // 2, max_batch=32, seed=11)
+    committee, valid, beta, msgs = run_btx(api, st, 32, invalid=(5,))
+    got = read_f12(api, msgs, 32)
+    g = O.gt_generator()
+    for i in range(32):
+        if i == 5:
+            assert got[i] == O.F12_ONE
+        else:
+            assert got[i] == O.f12_pow(g, st.s[i]), "slot %d" % i

@@ -174,3 +174,37 @@ def test_g1_msm_window_override_same_result(api, c):
     finally:
         api.set_msm_window(1, n, 0)
     assert g1_jac_affine(api.to_host(out)) == g1_mul_gen(sum(k * s for k, s in zip(ks, sc)))
+
+
+@pytest.mark.parametrize("group", [1, 2])
+def test_batched_msm_uneven_segments(api, group):
+    """Many MSMs of uneven sizes (0, 1, 5, 37, 200, 3 terms) in one launch sequence, each equal
+    to its closed form; G2 also with gathered (indexed) points."""
+    rnd = random.Random(600 + group)
+    sizes = [0, 1, 5, 37, 200, 3] if group == 1 else [0, 1, 5, 37, 3]
+    mul = g1_mul_gen if group == 1 else g2_mul_gen
+    pack = g1_points_bytes if group == 1 else g2_points_bytes
+    read = g1_jac_affine if group == 1 else g2_jac_affine
+    width = 144 if group == 1 else 288
+    npts = 40
+    ks, pts = _rand_bases(rnd, npts, mul)
+    nterms = sum(sizes)
+    idx = [rnd.randrange(npts) for _ in range(nterms)]
+    sc = [rnd.randrange(O.R) if rnd.random() < 0.8 else 1 << rnd.randrange(255) for _ in range(nterms)]
+    offs = [0]
+    for s in sizes:
+        offs.append(offs[-1] + s)
+    fn = api.g1_multi_exp_batched if group == 1 else api.g2_multi_exp_batched
+    out = api.to_host(fn(api.to_device(pack(pts)), api.to_device(pack_scalars(sc)), offs,
+                         torch.tensor(idx, dtype=torch.int32, device="cuda")))
+    for sgm in range(len(sizes)):
+        want = sum(ks[idx[t]] * sc[t] for t in range(offs[sgm], offs[sgm + 1]))
+        got = read(out[width * sgm: width * (sgm + 1)])
+        assert got == (mul(want) if want % O.R else None), "segment %d" % sgm
+    if group == 1:
+        # contiguous (non-gathered) points
+        pts2 = [pts[i] for i in idx]
+        out = api.to_host(fn(api.to_device(pack(pts2)), api.to_device(pack_scalars(sc)), offs))
+        for sgm in range(len(sizes)):
+            want = sum(ks[idx[t]] * sc[t] for t in range(offs[sgm], offs[sgm + 1]))
+            assert read(out[width * sgm: width * (sgm + 1)]) == (mul(want) if want % O.R else None)

@@ -32,6 +32,8 @@ _SIGNATURES = {
     "bte_g2_msm": [_vp, _vp, _sz, _vp, _vp],
     "bte_g2_msm_indexed": [_vp, _sz, _vp, _vp, _sz, _vp, _vp],
     "bte_msm_set_window": [_i, _sz, _i],
+    "bte_g1_msm_batched": [_vp, _sz, _vp, _vp, _sz, _vp, _sz, _vp, _vp],
+    "bte_g2_msm_batched": [_vp, _sz, _vp, _vp, _sz, _vp, _sz, _vp, _vp],
     "bte_g1_batch_normalize": [_vp, _sz, _vp, _vp],
     "bte_g2_batch_normalize": [_vp, _sz, _vp, _vp],
     "bte_g1_generator_mul": [_vp, _sz, _vp, _vp],
@@ -50,6 +52,11 @@ _SIGNATURES = {
     "bte_final_exp_easy": [_vp, _sz, _vp, _vp],
     "bte_final_exp_hard": [_vp, _sz, _vp, _vp],
     "bte_final_exp": [_vp, _sz, _vp, _vp],
+    "bte_lagrange_at_zero": [_sz, _vp, _sz, _vp],
+    "bte_btx_prepare": [_vp, _sz, _sz, _vp, _sz, _vp, _sz, _sz, _vp, _vp, _vp],
+    "bte_btx_precompute": [_vp, _vp, _sz, _vp, _vp, _vp],
+    "bte_btx_open": [_vp, _sz, _vp, _vp, _vp, _vp, _sz, _vp, _vp],
+    "bte_btx_partial_decrypt": [_vp, _sz, _vp, _sz, _vp, _vp],
     "bte_profile_enable": [_i],
     "bte_profile_read": [ctypes.c_char_p, _sz],
     "bte_probe_mac_rate": [ctypes.POINTER(ctypes.c_double), _vp],

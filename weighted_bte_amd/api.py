@@ -119,6 +119,38 @@ def g2_generator_mul(scalar_bytes):
     return out
 
 
+def _msm_batched(fn, width, jwidth, points, scalar_bytes, seg_offsets, indices):
+    points, scalar_bytes = _dev(points), _dev(scalar_bytes)
+    nterms = scalar_bytes.numel() // SCALAR
+    if indices is not None:
+        indices = _dev(indices)
+        if indices.numel() != nterms:
+            raise ValueError("indices.len() != scalars.len()")
+    elif points.numel() != nterms * width:
+        raise ValueError("points.len() != scalars.len()")
+    if seg_offsets[0] != 0 or seg_offsets[-1] != nterms or any(
+            b < a for a, b in zip(seg_offsets, seg_offsets[1:])):
+        raise ValueError("bad segment offsets")
+    offs = torch.tensor(seg_offsets, dtype=torch.int64, device=points.device)
+    nseg = len(seg_offsets) - 1
+    out = _empty(nseg * jwidth, points)
+    call(fn, _ptr(points), points.numel() // width, _ptr(indices), _ptr(scalar_bytes), nterms,
+         _ptr(offs), nseg, _ptr(out), _stream())
+    return out
+
+
+def g1_multi_exp_batched(points, scalar_bytes, seg_offsets, indices=None):
+    """One G1 MSM per segment of terms (rayon fan-out of g1_multi_exp_affine_bytes)."""
+    return _msm_batched("bte_g1_msm_batched", G1_AFFINE, G1_JAC, points, scalar_bytes, seg_offsets,
+                        indices)
+
+
+def g2_multi_exp_batched(points, scalar_bytes, seg_offsets, indices=None):
+    """One G2 MSM per segment of terms (rayon fan-out of g2_multi_exp_affine_bytes/_indexed)."""
+    return _msm_batched("bte_g2_msm_batched", G2_AFFINE, G2_JAC, points, scalar_bytes, seg_offsets,
+                        indices)
+
+
 def set_msm_window(group, n, bits):
     call("bte_msm_set_window", group, n, bits)
 
@@ -283,4 +315,64 @@ def full_final_exponentiation(values):
     v = _dev(values)
     out = torch.empty_like(v)
     call("bte_final_exp", _ptr(v), v.numel() // FP12, _ptr(out), _stream())
+    return out
+
+
+# --- a16: weighted_btx phases (weighted_btx/src/decryption.rs) ------------------------------
+def lagrange_at_zero(total_weight, selected):
+    """interpolation.rs:31-52 (host restatement in the library); returns canonical ints."""
+    sel = (ctypes.c_uint64 * len(selected))(*selected)
+    out = ctypes.create_string_buffer(32 * len(selected))
+    call("bte_lagrange_at_zero", total_weight, sel, len(selected), out)
+    raw = out.raw
+    return [int.from_bytes(raw[32 * i: 32 * i + 32], "little") for i in range(len(selected))]
+
+
+def btx_prepare(material, total_weight, max_batch, party_offsets, accepted, batch_size):
+    """prepare_decryption (decryption.rs:427-551) -> (kernel_lines, opening_keys)."""
+    material = _dev(material)
+    m = 1
+    while m < 2 * batch_size:
+        m <<= 1
+    offs = (ctypes.c_uint64 * len(party_offsets))(*party_offsets)
+    acc = (ctypes.c_uint32 * len(accepted))(*accepted)
+    lines = _empty(m * LINES_BYTES, material)
+    keys = _empty(batch_size * len(accepted) * G2_AFFINE, material)
+    call("bte_btx_prepare", _ptr(material), total_weight, max_batch, offs, len(party_offsets) - 1, acc,
+         len(accepted), batch_size, _ptr(lines), _ptr(keys), _stream())
+    return lines, keys
+
+
+def btx_precompute(ct1_affine, valid, kernel_lines):
+    """precompute_batch (decryption.rs:555-603) -> beta (B x Fp12)."""
+    ct1 = _dev(ct1_affine)
+    B = ct1.numel() // G1_AFFINE
+    v = (ctypes.c_uint8 * B)(*[1 if x else 0 for x in valid])
+    beta = _empty(B * FP12, ct1)
+    call("bte_btx_precompute", _ptr(ct1), v, B, _ptr(_dev(kernel_lines)), _ptr(beta), _stream())
+    return beta
+
+
+def btx_open(shares_jac, opening_keys, ct2, beta, valid):
+    """open_batch (decryption.rs:607-695) -> messages (B x Fp12; one where invalid)."""
+    shares = _dev(shares_jac)
+    T = shares.numel() // G1_JAC
+    ct2 = _dev(ct2)
+    B = ct2.numel() // FP12
+    v = (ctypes.c_uint8 * B)(*[1 if x else 0 for x in valid])
+    out = _empty(B * FP12, ct2)
+    call("bte_btx_open", _ptr(shares), T, _ptr(_dev(opening_keys)), _ptr(ct2), _ptr(_dev(beta)), v, B,
+         _ptr(out), _stream())
+    return out
+
+
+def btx_partial_decrypt(ct1_affine, q_scalars):
+    """partial_decrypt (decryption.rs:205-233) for every party key q_j -> sigma (N x G1 Jacobian)."""
+    ct1 = _dev(ct1_affine)
+    B = ct1.numel() // G1_AFFINE
+    N = len(q_scalars)
+    qb = b"".join(int(q).to_bytes(32, "little") for q in q_scalars)
+    buf = ctypes.create_string_buffer(qb, len(qb))
+    out = _empty(N * G1_JAC, ct1)
+    call("bte_btx_partial_decrypt", _ptr(ct1), B, buf, N, _ptr(out), _stream())
     return out

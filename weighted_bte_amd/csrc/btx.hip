@@ -1,0 +1,444 @@
+// weighted_btx batch decryption on the GPU: the host control flow of
+// weighted_btx/src/decryption.rs (prepare_decryption :427-551, precompute_batch :555-603,
+// open_batch :607-695, partial_decrypt :205-233) over this library's kernels, plus the host
+// restatement of lagrange_at_zero (weighted_btx/src/interpolation.rs:31-140).  Protocol
+// bookkeeping (proof checks, digests, setup and committee binding) stays with the caller; these
+// entry points take the already-validated arrays.
+//
+// Key material layout (device): one array of G2 affine points holding the reference's
+// negative_material (L blocks of W, block i-1 = [z(w_v) q_owner^-i]_2) followed by its
+// positive_material (L-1 blocks of W, block i-1 = [z(w_v) q_owner^i]_2), i.e.
+// material[(i-1) W + v] and material[L W + (i-1) W + v].
+#include <vector>
+
+#include "common.hpp"
+#include "curve.hpp"
+#include "cyclo.hpp"
+#include "host_field.hpp"
+
+extern "C" {
+int bte_fft_g1(bte_p1* values, size_t n, int inverse, int scaled, void* stream);
+int bte_fft_g2(bte_p2* values, size_t n, int inverse, int scaled, void* stream);
+int bte_ifft_cyclotomic_prefix_unscaled(bte_fp12* values, size_t n, size_t output_count, void* stream);
+int bte_g1_batch_normalize(const bte_p1* in, size_t n, bte_p1_affine* out, void* stream);
+int bte_g2_batch_normalize(const bte_p2* in, size_t n, bte_p2_affine* out, void* stream);
+int bte_g2_prepare_lines(const bte_p2_affine* q, size_t n, bte_fp6* lines, void* stream);
+int bte_miller_loop_prepared(const bte_fp6* lines, const bte_p1_affine* p, size_t n, bte_fp12* out, void* stream);
+int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine* q, size_t npairs, const uint64_t* seg_offsets,
+                          size_t nseg, bte_fp12* out, void* stream);
+int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
+int bte_final_exp_hard(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
+int bte_final_exp(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
+}
+
+namespace bte {
+
+// ---------------------------------------------------------------------------------------------
+// Host: Lagrange coefficients at zero (interpolation.rs:31-140).  The selected points are
+// x_v = w^v, w the generator of the radix-2 domain of size nextpow2(W).  The coefficients are
+// unique field values; this restatement uses L_v(0) = prod_{u != v} x_u / (x_u - x_v), with one
+// batched inversion, and the closed form 1/M for a complete domain (interpolation.rs:36-44).
+// ---------------------------------------------------------------------------------------------
+static int lagrange_at_zero(size_t total_weight, const std::vector<uint64_t>& sel, std::vector<bteh::Fr>& out) {
+  using namespace bteh;
+  const size_t n = sel.size();
+  if (total_weight == 0 || n == 0) return set_error(BTE_EINVAL, "InvalidInterpolationSet");
+  std::vector<char> seen(total_weight, 0);
+  for (uint64_t v : sel) {
+    if (v >= total_weight || seen[v]) return set_error(BTE_EINVAL, "InvalidInterpolationSet");
+    seen[v] = 1;
+  }
+  size_t dom = 1;
+  int logd = 0;
+  while (dom < total_weight) {
+    dom <<= 1;
+    ++logd;
+  }
+  out.assign(n, Fr{});
+  if (n == total_weight && dom == total_weight) {
+    Fr inv = fr_inv(fr_from_u64(total_weight));
+    for (auto& c : out) c = inv;
+    return BTE_OK;
+  }
+  Fr w;
+  memcpy(w.l, FR_ROOT_OF_UNITY, 32);
+  for (int i = logd; i < FR_S; ++i) w = fr_mul(w, w);
+  std::vector<Fr> x(n);
+  for (size_t i = 0; i < n; ++i) x[i] = fr_pow_u64(w, sel[i]);
+  // num_v = prod_{u != v} x_u ; den_v = prod_{u != v} (x_u - x_v)
+  Fr prod_all = fr_from_u64(1);
+  for (size_t i = 0; i < n; ++i) prod_all = fr_mul(prod_all, x[i]);
+  std::vector<Fr> den(n);
+  for (size_t v = 0; v < n; ++v) {
+    Fr d = x[v];  // folds the 1/x_v of num_v = prod_all / x_v into the denominator
+    for (size_t u = 0; u < n; ++u)
+      if (u != v) d = fr_mul(d, fr_sub(x[u], x[v]));
+    den[v] = d;
+  }
+  // batch inversion of den
+  std::vector<Fr> pre(n);
+  Fr acc = fr_from_u64(1);
+  for (size_t i = 0; i < n; ++i) {
+    pre[i] = acc;
+    acc = fr_mul(acc, den[i]);
+  }
+  Fr inv = fr_inv(acc);
+  for (size_t i = n; i-- > 0;) {
+    Fr di = fr_mul(inv, pre[i]);
+    inv = fr_mul(inv, den[i]);
+    out[i] = fr_mul(prod_all, di);
+  }
+  return BTE_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Glue kernels
+// ---------------------------------------------------------------------------------------------
+template <class F> __global__ void k_fill_identity(jac<F>* v, size_t n) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) v[i] = jac_identity<F>();
+}
+
+// out[s] = sum of in[s * len .. (s+1) * len), one 64-thread group per output.
+template <class F>
+__global__ void __launch_bounds__(64) k_sum_runs(const jac<F>* __restrict__ in, size_t len, size_t nout,
+                                                 jac<F>* __restrict__ out) {
+  __shared__ jac<F> sh[64];
+  const size_t s = blockIdx.x;
+  if (s >= nout) return;
+  const int tid = threadIdx.x;
+  jac<F> acc = jac_identity<F>();
+  for (size_t i = tid; i < len; i += 64) acc = jac_add(acc, in[s * len + i]);
+  sh[tid] = acc;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (tid < w) sh[tid] = jac_add(sh[tid], sh[tid + w]);
+    __syncthreads();
+  }
+  if (tid == 0) out[s] = sh[0];
+}
+
+// kernel[d] = neg[d-1] (d = 1..B-1), kernel[m-d] = pos[d-1] (decryption.rs:494-521).
+__global__ void k_place_kernel(const g2j* __restrict__ neg, const g2j* __restrict__ pos, size_t B, size_t m,
+                               g2j* __restrict__ kernel) {
+  const size_t d = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (d == 0 || d >= B) return;
+  kernel[d] = neg[d - 1];
+  kernel[m - d] = pos[d - 1];
+}
+
+__global__ void __launch_bounds__(64) k_scale_all_g2(g2j* __restrict__ v, size_t n, const uint32_t* __restrict__ s) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) v[i] = group_scalar_mul(v[i], s);
+}
+
+__global__ void k_g1_pad(const g1a* __restrict__ in, size_t B, size_t m, g1j* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  out[i] = i < B ? jac_from_aff(in[i]) : jac_identity<fp>();
+}
+
+template <class T> __global__ void k_tile(const T* __restrict__ in, size_t n, size_t reps, T* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n * reps) out[i] = in[i % n];
+}
+
+// beta for invalid slots is the identity (decryption.rs:585-594).
+__global__ void k_mask_invalid(fp12* __restrict__ v, const uint8_t* __restrict__ valid, size_t n) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n && !valid[i]) v[i] = fp12_one();
+}
+
+// m_i = ct2_i - alpha_i + beta_i in additive GT notation (decryption.rs:689-694).
+__global__ void __launch_bounds__(64) k_open_combine(const fp12* __restrict__ ct2, const fp12* __restrict__ alpha,
+                                                     const fp12* __restrict__ beta, const uint8_t* __restrict__ valid,
+                                                     size_t n, fp12* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (!valid[i]) {
+    out[i] = fp12_one();
+    return;
+  }
+  out[i] = fp12_mul(fp12_mul(ct2[i], fp12_conj(alpha[i])), beta[i]);
+}
+
+// Stream-ordered scratch that is released when the driver returns.
+struct Scratch {
+  hipStream_t s;
+  std::vector<void*> ptrs;
+  explicit Scratch(hipStream_t st) : s(st) {}
+  ~Scratch() {
+    for (void* p : ptrs) (void)hipFreeAsync(p, s);
+  }
+  template <class T> int get(T** p, size_t count) {
+    hipError_t e = hipMallocAsync((void**)p, (count ? count : 1) * sizeof(T), s);
+    if (e != hipSuccess) return set_error(BTE_ENOMEM, "scratch (%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    ptrs.push_back((void*)*p);
+    return BTE_OK;
+  }
+  template <class T> int upload(T** p, const T* host, size_t count) {
+    int rc = get(p, count);
+    if (rc) return rc;
+    BTE_HIP(hipMemcpyAsync(*p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return BTE_OK;
+  }
+  // Host sources of upload() may be temporaries: wait until the copies have consumed them.
+  int uploads_done() {
+    BTE_HIP(hipStreamSynchronize(s));
+    return BTE_OK;
+  }
+};
+
+#define BTE_TRY(x)        \
+  do {                    \
+    int rc_ = (x);        \
+    if (rc_) return rc_;  \
+  } while (0)
+
+static size_t next_pow2(size_t v) {
+  size_t m = 1;
+  while (m < v) m <<= 1;
+  return m;
+}
+
+}  // namespace bte
+
+using namespace bte;
+
+// interpolation.rs:31-52 lagrange_at_zero over the first W points of the radix-2 domain of size
+// nextpow2(W); host memory: selected (n u64 indices), out (n canonical 32-byte scalars).
+extern "C" int bte_lagrange_at_zero(size_t total_weight, const uint64_t* selected, size_t n, uint8_t* out) {
+  if (!selected || !out) return set_error(BTE_EINVAL, "lagrange_at_zero: null pointer");
+  std::vector<uint64_t> sel(selected, selected + n);
+  std::vector<bteh::Fr> c;
+  BTE_TRY(lagrange_at_zero(total_weight, sel, c));
+  memcpy(out, c.data(), 32 * n);
+  return BTE_OK;
+}
+
+// decryption.rs:427-551 prepare_decryption.  Host inputs: party_offsets (N+1), accepted party
+// indices (T, in the committee order), batch size B <= L.  Device outputs: kernel_lines
+// (m = nextpow2(2B) tables of 68 lines, the transformed kernel scaled by 1/m) and opening_keys
+// (B x T G2 affine, slot-major).
+extern "C" int bte_btx_prepare(const bte_p2_affine* material, size_t total_weight, size_t max_batch,
+                               const uint64_t* party_offsets, size_t party_count, const uint32_t* accepted,
+                               size_t n_accepted, size_t batch_size, bte_fp6* kernel_lines,
+                               bte_p2_affine* opening_keys, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t W = total_weight, L = max_batch, B = batch_size, T = n_accepted;
+  if (!material || !party_offsets || !accepted || !kernel_lines || !opening_keys)
+    return set_error(BTE_EINVAL, "btx_prepare: null pointer");
+  if (B == 0 || B > L) return set_error(BTE_EINVAL, "btx_prepare: InvalidBatchSize(%zu)", B);
+  if (T == 0) return set_error(BTE_EINVAL, "btx_prepare: empty committee");
+  // selected virtual indices in committee order and per-party offsets into them
+  std::vector<uint64_t> sel;
+  std::vector<uint64_t> coff(1, 0);
+  for (size_t p = 0; p < T; ++p) {
+    const uint32_t j = accepted[p];
+    if (j >= party_count) return set_error(BTE_EINVAL, "btx_prepare: InvalidPartyIndex(%u)", j);
+    for (uint64_t v = party_offsets[j]; v < party_offsets[j + 1]; ++v) sel.push_back(v);
+    coff.push_back(sel.size());
+  }
+  if (party_offsets[party_count] != W) return set_error(BTE_EINVAL, "btx_prepare: offsets do not sum to W");
+  const size_t WT = sel.size();
+  std::vector<bteh::Fr> lam;
+  BTE_TRY(lagrange_at_zero(W, sel, lam));
+  const size_t m = next_pow2(2 * B);
+
+  // Negative MSMs: segment (slot, party p) over negative_power(slot+1)[range_p] with lambda.
+  std::vector<uint32_t> nidx(B * WT), nsidx(B * WT);
+  std::vector<uint64_t> noff(B * T + 1);
+  for (size_t slot = 0; slot < B; ++slot)
+    for (size_t p = 0; p < T; ++p) {
+      noff[slot * T + p] = slot * WT + coff[p];
+      for (uint64_t k = coff[p]; k < coff[p + 1]; ++k) {
+        nidx[slot * WT + k] = (uint32_t)(slot * W + sel[k]);
+        nsidx[slot * WT + k] = (uint32_t)k;
+      }
+    }
+  noff[B * T] = B * WT;
+  // Positive MSMs: segment d-1 over positive_power(d) at the selected indices (d = 1..B-1).
+  const size_t P = B - 1;
+  std::vector<uint32_t> pidx(P * WT), psidx(P * WT);
+  std::vector<uint64_t> poff(P + 1);
+  for (size_t d = 1; d <= P; ++d) {
+    poff[d - 1] = (d - 1) * WT;
+    for (size_t k = 0; k < WT; ++k) {
+      pidx[(d - 1) * WT + k] = (uint32_t)(L * W + (d - 1) * W + sel[k]);
+      psidx[(d - 1) * WT + k] = (uint32_t)k;
+    }
+  }
+  poff[P] = P * WT;
+  const size_t n_points = L * W + (L - 1) * W;
+  if (n_points > 0xffffffffu) return set_error(BTE_EINVAL, "btx_prepare: material too large for 32-bit indices");
+
+  Scratch sc(s);
+  uint8_t* d_lam;
+  uint32_t *d_nidx, *d_nsidx, *d_pidx = nullptr, *d_psidx = nullptr;
+  uint64_t *d_noff, *d_poff = nullptr;
+  g2j *opening, *negsum, *possum, *kernel;
+  g2a* kernel_aff;
+  uint32_t* d_minv;
+  BTE_TRY(sc.upload(&d_lam, (const uint8_t*)lam.data(), 32 * WT));
+  BTE_TRY(sc.upload(&d_nidx, nidx.data(), nidx.size()));
+  BTE_TRY(sc.upload(&d_nsidx, nsidx.data(), nsidx.size()));
+  BTE_TRY(sc.upload(&d_noff, noff.data(), noff.size()));
+  if (P) {
+    BTE_TRY(sc.upload(&d_pidx, pidx.data(), pidx.size()));
+    BTE_TRY(sc.upload(&d_psidx, psidx.data(), psidx.size()));
+    BTE_TRY(sc.upload(&d_poff, poff.data(), poff.size()));
+  }
+  BTE_TRY(sc.get(&opening, B * T));
+  BTE_TRY(sc.get(&negsum, P ? P : 1));
+  BTE_TRY(sc.get(&possum, P ? P : 1));
+  BTE_TRY(sc.get(&kernel, m));
+  BTE_TRY(sc.get(&kernel_aff, m));
+  bteh::Fr minv = bteh::fr_inv(bteh::fr_from_u64(m));
+  BTE_TRY(sc.upload(&d_minv, (const uint32_t*)minv.l, 8));
+  BTE_TRY(sc.uploads_done());
+
+  PhaseTimer pt(s);
+  pt.begin();
+  BTE_TRY(g2_msm_batched_gather(material, n_points, d_nidx, d_lam, d_nsidx, B * WT, d_noff, B * T, 0, opening, s));
+  if (P) BTE_TRY(g2_msm_batched_gather(material, n_points, d_pidx, d_lam, d_psidx, P * WT, d_poff, P, 0, possum, s));
+  pt.mark("g2_msms");
+  // kernel[d] = sum_p opening[(d-1) T + p]
+  if (P) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sum_runs<fp2>), dim3((unsigned)P), dim3(64), 0, s, (const g2j*)opening, T, P, negsum);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fill_identity<fp2>), dim3(blocks_for(m, 256)), dim3(256), 0, s, kernel, m);
+  if (P) hipLaunchKernelGGL(k_place_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, s, (const g2j*)negsum, (const g2j*)possum, B, m, kernel);
+  BTE_TRY(bte_fft_g2((bte_p2*)kernel, m, 0, 0, s));
+  hipLaunchKernelGGL(k_scale_all_g2, dim3(blocks_for(m, 64)), dim3(64), 0, s, kernel, m, (const uint32_t*)d_minv);
+  pt.mark("kernel_fft");
+  BTE_TRY(bte_g2_batch_normalize((const bte_p2*)kernel, m, (bte_p2_affine*)kernel_aff, s));
+  BTE_TRY(bte_g2_prepare_lines((const bte_p2_affine*)kernel_aff, m, kernel_lines, s));
+  BTE_TRY(bte_g2_batch_normalize((const bte_p2*)opening, B * T, opening_keys, s));
+  pt.mark("lines");
+  BTE_CHECK_LAUNCH();
+  if (pt.on) {
+    BTE_HIP(hipStreamSynchronize(s));
+    pt.publish();
+  }
+  return BTE_OK;
+}
+
+// decryption.rs:555-603 precompute_batch.  ct1: B G1 affine first components (identity for
+// invalid slots), valid: B flags (host).  beta_out: B Fp12 on the device.
+extern "C" int bte_btx_precompute(const bte_p1_affine* ct1, const uint8_t* valid, size_t batch_size,
+                                  const bte_fp6* kernel_lines, bte_fp12* beta_out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t B = batch_size;
+  if (!ct1 || !valid || !kernel_lines || !beta_out) return set_error(BTE_EINVAL, "btx_precompute: null pointer");
+  if (B == 0) return set_error(BTE_EINVAL, "btx_precompute: InvalidBatchSize(0)");
+  const size_t m = next_pow2(2 * B);
+  Scratch sc(s);
+  g1j* c;
+  g1a* ca;
+  fp12 *ml, *conv;
+  uint8_t* d_valid;
+  BTE_TRY(sc.get(&c, m));
+  BTE_TRY(sc.get(&ca, m));
+  BTE_TRY(sc.get(&ml, m));
+  BTE_TRY(sc.get(&conv, m));
+  BTE_TRY(sc.upload(&d_valid, valid, B));
+  BTE_TRY(sc.uploads_done());
+  PhaseTimer pt(s);
+  pt.begin();
+  hipLaunchKernelGGL(k_g1_pad, dim3(blocks_for(m, 256)), dim3(256), 0, s, (const g1a*)ct1, B, m, c);
+  BTE_TRY(bte_fft_g1((bte_p1*)c, m, 0, 0, s));
+  BTE_TRY(bte_g1_batch_normalize((const bte_p1*)c, m, (bte_p1_affine*)ca, s));
+  pt.mark("g1_fft");
+  BTE_TRY(bte_miller_loop_prepared(kernel_lines, (const bte_p1_affine*)ca, m, (bte_fp12*)ml, s));
+  pt.mark("miller");
+  BTE_TRY(bte_final_exp_easy((const bte_fp12*)ml, m, (bte_fp12*)conv, s));
+  pt.mark("easy");
+  BTE_TRY(bte_ifft_cyclotomic_prefix_unscaled((bte_fp12*)conv, m, B, s));
+  pt.mark("cyclo_ifft");
+  BTE_TRY(bte_final_exp_hard((const bte_fp12*)conv, B, beta_out, s));
+  hipLaunchKernelGGL(k_mask_invalid, dim3(blocks_for(B, 256)), dim3(256), 0, s, (fp12*)beta_out, (const uint8_t*)d_valid, B);
+  pt.mark("hard");
+  BTE_CHECK_LAUNCH();
+  if (pt.on) {
+    BTE_HIP(hipStreamSynchronize(s));
+    pt.publish();
+  }
+  return BTE_OK;
+}
+
+// decryption.rs:607-695 open_batch.  shares: T G1 Jacobian (committee order), opening_keys from
+// bte_btx_prepare, ct2 / beta: B Fp12, valid: B flags (host).  messages_out: B Fp12; invalid
+// slots (the reference's None) are written as one.
+extern "C" int bte_btx_open(const bte_p1* shares, size_t n_accepted, const bte_p2_affine* opening_keys,
+                            const bte_fp12* ct2, const bte_fp12* beta, const uint8_t* valid, size_t batch_size,
+                            bte_fp12* messages_out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t B = batch_size, T = n_accepted;
+  if (!shares || !opening_keys || !ct2 || !beta || !valid || !messages_out)
+    return set_error(BTE_EINVAL, "btx_open: null pointer");
+  if (B == 0 || T == 0) return set_error(BTE_EINVAL, "btx_open: empty batch or committee");
+  Scratch sc(s);
+  g1a *sa, *tiled;
+  fp12 *ml, *alpha;
+  uint64_t* d_off;
+  uint8_t* d_valid;
+  std::vector<uint64_t> off(B + 1);
+  for (size_t i = 0; i <= B; ++i) off[i] = i * T;
+  BTE_TRY(sc.get(&sa, T));
+  BTE_TRY(sc.get(&tiled, B * T));
+  BTE_TRY(sc.get(&ml, B));
+  BTE_TRY(sc.get(&alpha, B));
+  BTE_TRY(sc.upload(&d_off, off.data(), B + 1));
+  BTE_TRY(sc.upload(&d_valid, valid, B));
+  BTE_TRY(sc.uploads_done());
+  PhaseTimer pt(s);
+  pt.begin();
+  BTE_TRY(bte_g1_batch_normalize(shares, T, (bte_p1_affine*)sa, s));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile<g1a>), dim3(blocks_for(B * T, 256)), dim3(256), 0, s, (const g1a*)sa, T, B, tiled);
+  BTE_TRY(bte_multi_miller_loop((const bte_p1_affine*)tiled, opening_keys, B * T, d_off, B, (bte_fp12*)ml, s));
+  pt.mark("multi_miller");
+  BTE_TRY(bte_final_exp((const bte_fp12*)ml, B, (bte_fp12*)alpha, s));
+  pt.mark("final_exp");
+  hipLaunchKernelGGL(k_open_combine, dim3(blocks_for(B, 64)), dim3(64), 0, s, (const fp12*)ct2, (const fp12*)alpha,
+                     (const fp12*)beta, (const uint8_t*)d_valid, B, (fp12*)messages_out);
+  pt.mark("combine");
+  BTE_CHECK_LAUNCH();
+  if (pt.on) {
+    BTE_HIP(hipStreamSynchronize(s));
+    pt.publish();
+  }
+  return BTE_OK;
+}
+
+// decryption.rs:205-233 partial_decrypt for every party at once: sigma_j = sum_i q_j^i C_i
+// (i = 1..B).  q: N canonical scalars (host); ct1: B G1 affine (device); sigma_out: N Jacobian.
+extern "C" int bte_btx_partial_decrypt(const bte_p1_affine* ct1, size_t batch_size, const uint8_t* q, size_t n_parties,
+                                       bte_p1* sigma_out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t B = batch_size, N = n_parties;
+  if (!ct1 || !q || !sigma_out) return set_error(BTE_EINVAL, "btx_partial_decrypt: null pointer");
+  if (N == 0) return BTE_OK;
+  std::vector<bteh::Fr> pw(N * B);
+  std::vector<uint32_t> pidx(N * B);
+  std::vector<uint64_t> off(N + 1);
+  for (size_t j = 0; j < N; ++j) {
+    bteh::Fr qj;
+    memcpy(qj.l, q + 32 * j, 32);
+    bteh::Fr cur = qj;
+    off[j] = j * B;
+    for (size_t i = 0; i < B; ++i) {
+      pw[j * B + i] = cur;
+      pidx[j * B + i] = (uint32_t)i;
+      cur = bteh::fr_mul(cur, qj);
+    }
+  }
+  off[N] = N * B;
+  Scratch sc(s);
+  uint8_t* d_pw;
+  uint32_t* d_pidx;
+  uint64_t* d_off;
+  BTE_TRY(sc.upload(&d_pw, (const uint8_t*)pw.data(), 32 * N * B));
+  BTE_TRY(sc.upload(&d_pidx, pidx.data(), N * B));
+  BTE_TRY(sc.upload(&d_off, off.data(), N + 1));
+  BTE_TRY(sc.uploads_done());
+  BTE_TRY(g1_msm_batched_gather(ct1, B, d_pidx, d_pw, nullptr, N * B, d_off, N, 0, sigma_out, s));
+  return BTE_OK;
+}

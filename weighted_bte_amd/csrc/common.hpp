@@ -37,6 +37,14 @@ struct PhaseTimer {
   void publish();               // call after the stream is synchronised
 };
 
+// Batched MSMs with gathered points and scalars (msm.hip); c = 0 picks the window width.
+int g1_msm_batched_gather(const void* points, size_t n_points, const uint32_t* point_index, const uint8_t* scalars,
+                          const uint32_t* scalar_index, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
+                          int c, void* out, hipStream_t s);
+int g2_msm_batched_gather(const void* points, size_t n_points, const uint32_t* point_index, const uint8_t* scalars,
+                          const uint32_t* scalar_index, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
+                          int c, void* out, hipStream_t s);
+
 inline unsigned blocks_for(size_t n, unsigned threads) {
   return (unsigned)((n + threads - 1) / threads);
 }

@@ -151,4 +151,17 @@ template <class F> BTE_DEV jac<F> jac_mul_scalar(const jac<F>& p, const uint32_t
   return acc;
 }
 
+// Left-to-right double-and-add by a canonical scalar (8 x u32, below 2^255).
+template <class F> BTE_DEV jac<F> group_scalar_mul(const jac<F>& p, const uint32_t* __restrict__ k) {
+  jac<F> acc = jac_identity<F>();
+  int top = 255;
+  while (top >= 0 && !((k[top >> 5] >> (top & 31)) & 1u)) --top;
+#pragma unroll 1
+  for (int i = top; i >= 0; --i) {
+    acc = jac_dbl(acc);
+    if ((k[i >> 5] >> (i & 31)) & 1u) acc = jac_add(acc, p);
+  }
+  return acc;
+}
+
 }  // namespace bte

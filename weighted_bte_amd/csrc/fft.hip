@@ -152,19 +152,6 @@ template <class T> __global__ void k_bit_reverse(T* v, size_t n, int logn) {
   }
 }
 
-// Left-to-right double-and-add by a canonical scalar (8 x u32, below 2^255).
-template <class F> BTE_DEV jac<F> group_scalar_mul(const jac<F>& p, const uint32_t* __restrict__ k) {
-  jac<F> acc = jac_identity<F>();
-  int top = 255;
-  while (top >= 0 && !((k[top >> 5] >> (top & 31)) & 1u)) --top;
-#pragma unroll 1
-  for (int i = top; i >= 0; --i) {
-    acc = jac_dbl(acc);
-    if ((k[i >> 5] >> (i & 31)) & 1u) acc = jac_add(acc, p);
-  }
-  return acc;
-}
-
 // One DIT stage for butterfly size 2 * 2^log_half; twiddle index k << (logn - 1 - log_half).
 template <class F>
 __global__ void __launch_bounds__(64) k_group_stage(jac<F>* __restrict__ v, size_t n, int logn, int log_half,

@@ -337,6 +337,84 @@ __global__ void __launch_bounds__(T) k_msm_window_sums(const jac<F>* __restrict_
 }
 
 // ---------------------------------------------------------------------------------------------
+// Batched (segmented) MSM: many independent MSMs of uneven sizes through one pipeline.  Term t
+// belongs to segment seg(t) (seg_offsets); its window digits go to bucket
+// (seg * nw + w) * M + |d| - 1, so the sort / accumulate / fix-up kernels above run unchanged
+// over all segments at once.  Each (segment, window) then reduces its M buckets with one
+// running sum, and one thread per segment combines the windows (Horner, c doublings per
+// window).  Nothing synchronises with the host, so the sequence can be graph-captured.
+// ---------------------------------------------------------------------------------------------
+static constexpr uint32_t NO_DIGIT = 0xffffffffu;
+
+__global__ void k_msm_digits_seg(const uint32_t* __restrict__ scalars, const uint32_t* __restrict__ scalar_index,
+                                 size_t nterms,
+                                 const uint64_t* __restrict__ seg_offsets, uint32_t nseg, int c, int nw,
+                                 uint32_t M, uint32_t* __restrict__ keys, uint32_t* __restrict__ counts) {
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= nterms) return;
+  uint32_t lo = 0, hi = nseg;  // seg_offsets[lo] <= t < seg_offsets[hi]
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (seg_offsets[mid] <= t) lo = mid; else hi = mid;
+  }
+  const uint32_t seg = lo;
+  const size_t si = scalar_index ? scalar_index[t] : t;
+  uint32_t s[9];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s[k] = scalars[8 * si + k];
+  s[7] &= 0x7fffffffu;
+  s[8] = 0;
+  const uint32_t mask = (1u << c) - 1u;
+  int carry = 0;
+  for (int w = 0; w < nw; ++w) {
+    const int bit = w * c;
+    const int word = bit >> 5, off = bit & 31;
+    uint64_t two = (uint64_t)s[word] | ((uint64_t)s[word + 1] << 32);
+    int d = (int)((two >> off) & mask) + carry;
+    if (d > (int)M) {
+      d -= (int)(1u << c);
+      carry = 1;
+    } else {
+      carry = 0;
+    }
+    uint32_t e = NO_DIGIT;
+    if (d) {
+      const uint32_t key = (seg * (uint32_t)nw + (uint32_t)w) * M + (uint32_t)(d < 0 ? -d : d) - 1u;
+      atomicAdd(&counts[key], 1u);
+      e = (key << 1) | (d < 0 ? 1u : 0u);
+    }
+    keys[(size_t)w * nterms + t] = e;
+  }
+}
+
+__global__ void k_msm_scatter_keys(const uint32_t* __restrict__ keys, size_t nterms, int nw,
+                                   const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
+                                   uint32_t* __restrict__ sorted) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= nterms * (size_t)nw) return;
+  const uint32_t e = keys[i];
+  if (e == NO_DIGIT) return;
+  const uint32_t key = e >> 1;
+  const uint32_t term = (uint32_t)(i % nterms);
+  const uint32_t pos = offsets[key] + atomicAdd(&cursor[key], 1u);
+  sorted[pos] = term | ((e & 1u) << 31);
+}
+
+template <class F>
+__global__ void __launch_bounds__(64) k_msm_combine(const jac<F>* __restrict__ win, uint32_t nseg, int nw, int c,
+                                                    jac<F>* __restrict__ out) {
+  const uint32_t sgm = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sgm >= nseg) return;
+  jac<F> acc = jac_identity<F>();
+  for (int w = nw - 1; w >= 0; --w) {
+    if (!jac_is_identity(acc))
+      for (int k = 0; k < c; ++k) acc = jac_dbl(acc);
+    acc = jac_add(acc, win[(size_t)sgm * nw + w]);
+  }
+  out[sgm] = acc;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Host driver
 // ---------------------------------------------------------------------------------------------
 // Grow-only device workspace per (device, stream).  An MSM call holds its entry's lock from
@@ -366,10 +444,14 @@ struct Carver {
   template <class T> void add(T** p, size_t count) {
     items.push_back({(void**)p, (count ? count : 1) * sizeof(T)});
   }
+  hipStream_t stream;
+  explicit Carver(hipStream_t s) : stream(s) {}
   int commit(Workspace* ws) {
     size_t total = 0;
     for (auto& it : items) total += (it.second + 255) & ~(size_t)255;
     if (total > ws->cap) {
+      // earlier asynchronous calls on this stream may still use the old buffer
+      BTE_HIP(hipStreamSynchronize(stream));
       if (ws->base) BTE_HIP(hipFree(ws->base));
       ws->base = nullptr;
       ws->cap = 0;
@@ -410,7 +492,7 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
 
   Workspace* ws = workspace_for(s);
   std::lock_guard<std::mutex> ws_lock(ws->mu);
-  Carver cv;
+  Carver cv(s);
   int32_t* digits;
   uint32_t *counts, *offsets, *cursor, *sorted, *tile_sums;
   jac<F> *buckets, *heads, *tails, *seg_acc, *seg_run, *wsum;
@@ -496,6 +578,90 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
   return BTE_OK;
 }
 
+
+// Window width for a batch from its mean segment size (the bucket reduction costs ~2^c per
+// window and segment, the accumulation one addition per term and window).
+static int batched_window(size_t nterms, size_t nseg) {
+  const size_t avg = nseg ? (nterms + nseg - 1) / nseg : nterms;
+  int c = floor_log2(avg ? avg : 1) - 1;
+  return std::max(3, std::min(12, c));
+}
+
+template <class F>
+static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t* index, const uint8_t* scalars,
+                           const uint32_t* scalar_index, size_t nterms, const uint64_t* seg_offsets, size_t nseg, int c_force, jac<F>* out,
+                           hipStream_t s) {
+  if (nseg == 0) return BTE_OK;
+  if (!seg_offsets || !out || (nterms && (!points || !scalars)))
+    return set_error(BTE_EINVAL, "msm_batched: null pointer");
+  if (((uintptr_t)scalars & 3) != 0) return set_error(BTE_EINVAL, "msm_batched: scalars must be 4-byte aligned");
+  if (nterms > 0x7fffffffu || nseg > 0x7fffffffu) return set_error(BTE_EINVAL, "msm_batched: too many terms");
+  const int c = c_force > 0 ? c_force : batched_window(nterms, nseg);
+  const int nw = (256 + c - 1) / c;
+  const uint32_t M = 1u << (c - 1);
+  const size_t nb64 = (size_t)nseg * nw * M;
+  if (nb64 >= 0x7fffffffu) return set_error(BTE_EINVAL, "msm_batched: %zu buckets exceed the index range", nb64);
+  const uint32_t nb = (uint32_t)nb64;
+  const size_t E_max = nterms * (size_t)nw;
+  const size_t nchunks = (E_max + MSM_CHUNK - 1) / MSM_CHUNK;
+  const int ntiles = (int)((nb + 1023) / 1024);
+  const size_t max_big = nchunks / FIX_SERIAL + 1;
+
+  Workspace* ws = workspace_for(s);
+  std::lock_guard<std::mutex> ws_lock(ws->mu);
+  Carver cv(s);
+  uint32_t *keys, *counts, *offsets, *cursor, *sorted, *tile_sums, *big, *nbig;
+  jac<F> *buckets, *heads, *tails, *win, *runs;
+  int* err;
+  cv.add(&keys, E_max);
+  cv.add(&counts, nb);
+  cv.add(&offsets, (size_t)nb + 1);
+  cv.add(&cursor, nb);
+  cv.add(&sorted, E_max);
+  cv.add(&tile_sums, ntiles);
+  cv.add(&buckets, nb);
+  cv.add(&heads, nchunks);
+  cv.add(&tails, nchunks);
+  cv.add(&win, (size_t)nseg * nw);
+  cv.add(&runs, (size_t)nseg * nw);
+  cv.add(&err, 1);
+  cv.add(&big, max_big);
+  cv.add(&nbig, 1);
+  int rc = cv.commit(ws);
+  if (rc) return rc;
+
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, counts, (size_t)nb, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, cursor, (size_t)nb, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)err, (size_t)1, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nbig, (size_t)1, 0u);
+  if (nterms)
+    hipLaunchKernelGGL(k_msm_digits_seg, dim3(blocks_for(nterms, 256)), dim3(256), 0, s, (const uint32_t*)scalars,
+                       scalar_index, nterms, seg_offsets, (uint32_t)nseg, c, nw, M, keys, counts);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(256), 0, s, counts, (size_t)nb, offsets, tile_sums);
+  hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(256), 0, s, tile_sums, ntiles, offsets + nb);
+  hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(nb, 256)), dim3(256), 0, s, offsets, (size_t)nb,
+                     (const uint32_t*)tile_sums);
+  if (E_max) {
+    hipLaunchKernelGGL(k_msm_scatter_keys, dim3(blocks_for(E_max, 256)), dim3(256), 0, s, (const uint32_t*)keys,
+                       nterms, nw, (const uint32_t*)offsets, cursor, sorted);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_accumulate<F>), dim3(blocks_for(nchunks, 64)), dim3(64), 0, s,
+                       points, index, index ? n_points : nterms, (const uint32_t*)sorted, (const uint32_t*)offsets,
+                       nb, buckets, heads, tails, err);
+  }
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup<F>), dim3(blocks_for(nb, 64)), dim3(64), 0, s,
+                     (const uint32_t*)offsets, nb, buckets, (const jac<F>*)heads, (const jac<F>*)tails, big, nbig);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_big<F>), dim3((unsigned)max_big), dim3(64), 0, s,
+                     (const uint32_t*)offsets, buckets, (const jac<F>*)heads, (const jac<F>*)tails,
+                     (const uint32_t*)big, (const uint32_t*)nbig);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_segsum<F>), dim3(blocks_for((size_t)nseg * nw, 64)), dim3(64), 0, s,
+                     (const jac<F>*)buckets, (int)(nseg * nw), M, c - 1, 0, win, runs);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_combine<F>), dim3(blocks_for(nseg, 64)), dim3(64), 0, s,
+                     (const jac<F>*)win, (uint32_t)nseg, nw, c, out);
+  BTE_CHECK_LAUNCH();
+  // The workspace is reused by the next call on this stream; stream order protects it.
+  return BTE_OK;
+}
+
 static_assert(sizeof(bteh::Jac<bteh::Fp>) == sizeof(bte_p1), "layout");
 static_assert(sizeof(bteh::Jac<bteh::Fp2>) == sizeof(bte_p2), "layout");
 static_assert(sizeof(jac<fp>) == sizeof(bte_p1), "layout");
@@ -532,3 +698,35 @@ extern "C" int bte_msm_set_window(int g1_or_g2, size_t n, int window_bits) {
   g_window_override[g1_or_g2 - 1][floor_log2(n)] = window_bits;
   return BTE_OK;
 }
+
+extern "C" int bte_g1_msm_batched(const bte_p1_affine* points, size_t n_points, const uint32_t* indices,
+                                  const uint8_t* scalars, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
+                                  bte_p1* out, void* stream) {
+  return msm_batched_run<fp>((const aff<fp>*)points, n_points, indices, scalars, nullptr, nterms, seg_offsets, nseg, 0,
+                             (jac<fp>*)out, (hipStream_t)stream);
+}
+
+extern "C" int bte_g2_msm_batched(const bte_p2_affine* points, size_t n_points, const uint32_t* indices,
+                                  const uint8_t* scalars, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
+                                  bte_p2* out, void* stream) {
+  return msm_batched_run<fp2>((const aff<fp2>*)points, n_points, indices, scalars, nullptr, nterms, seg_offsets, nseg, 0,
+                              (jac<fp2>*)out, (hipStream_t)stream);
+}
+
+// Batched MSMs whose terms gather both the point (points[point_index[t]]) and the scalar
+// (scalars[scalar_index[t]]); used by the scheme drivers to reuse one coefficient vector across
+// many bases without replicating it.
+namespace bte {
+int g1_msm_batched_gather(const void* points, size_t n_points, const uint32_t* point_index, const uint8_t* scalars,
+                          const uint32_t* scalar_index, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
+                          int c, void* out, hipStream_t s) {
+  return msm_batched_run<fp>((const aff<fp>*)points, n_points, point_index, scalars, scalar_index, nterms,
+                             seg_offsets, nseg, c, (jac<fp>*)out, s);
+}
+int g2_msm_batched_gather(const void* points, size_t n_points, const uint32_t* point_index, const uint8_t* scalars,
+                          const uint32_t* scalar_index, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
+                          int c, void* out, hipStream_t s) {
+  return msm_batched_run<fp2>((const aff<fp2>*)points, n_points, point_index, scalars, scalar_index, nterms,
+                              seg_offsets, nseg, c, (jac<fp2>*)out, s);
+}
+}  // namespace bte
