@@ -101,6 +101,12 @@ int bte_fft_g1(bte_p1* values, size_t n, int inverse, int scaled, void* stream);
 int bte_fft_g2(bte_p2* values, size_t n, int inverse, int scaled, void* stream);
 /* same over Gt (fft.rs:197-234 generic path, fft.rs:242-274 ifft_gt_unscaled) */
 int bte_fft_gt(bte_fp12* values, size_t n, int inverse, int scaled, void* stream);
+/* `count` independent transforms of size n stored back to back, one launch per stage */
+int bte_fft_g1_batched(bte_p1* values, size_t n, size_t count, int inverse, int scaled, void* stream);
+int bte_fft_g2_batched(bte_p2* values, size_t n, size_t count, int inverse, int scaled, void* stream);
+int bte_fft_cyclotomic_batched(bte_fp12* values, size_t n, size_t count, int inverse, void* stream);
+int bte_ifft_cyclotomic_prefix_unscaled_batched(bte_fp12* values, size_t n, size_t count,
+                                                size_t output_count, void* stream);
 /* weighted_pfe/src/fft.rs:324-331 fft_cyclotomic / ifft_cyclotomic_unscaled */
 int bte_fft_cyclotomic(bte_fp12* values, size_t n, int inverse, void* stream);
 /* weighted_btx/src/fft.rs:284-350 ifft_cyclotomic_prefix_unscaled: values[0..output_count)
@@ -123,6 +129,9 @@ int bte_g2_prepare_lines(const bte_p2_affine* q, size_t n, bte_fp6* lines, void*
 /* final_exponentiation.rs:207-219 PreparedG2Lines::miller_loop, pair i = (p[i], lines[68 i..]) */
 int bte_miller_loop_prepared(const bte_fp6* lines, const bte_p1_affine* p, size_t n, bte_fp12* out,
                              void* stream);
+/* as bte_miller_loop_prepared with pair i using table i % ntables */
+int bte_miller_loop_prepared_tiled(const bte_fp6* lines, size_t ntables, const bte_p1_affine* p,
+                                   size_t n, bte_fp12* out, void* stream);
 /* one unprepared Miller loop per pair (p[i], q[i]) */
 int bte_miller_loop_pairs(const bte_p1_affine* p, const bte_p2_affine* q, size_t n, bte_fp12* out,
                           void* stream);
@@ -168,6 +177,16 @@ int bte_btx_precompute(const bte_p1_affine* ct1, const uint8_t* valid, size_t ba
 int bte_btx_open(const bte_p1* shares, size_t n_accepted, const bte_p2_affine* opening_keys,
                  const bte_fp12* ct2, const bte_fp12* beta, const uint8_t* valid, size_t batch_size,
                  bte_fp12* messages_out, void* stream);
+
+/* Fused forms over `count` batches of one committee and batch size (the north-star batching):
+ * arrays hold count x B slots (count x T shares) back to back; one launch per stage serves all
+ * batches. */
+int bte_btx_precompute_batched(const bte_p1_affine* ct1, const uint8_t* valid, size_t batch_size,
+                               size_t count, const bte_fp6* kernel_lines, bte_fp12* beta_out,
+                               void* stream);
+int bte_btx_open_batched(const bte_p1* shares, size_t n_accepted, const bte_p2_affine* opening_keys,
+                         const bte_fp12* ct2, const bte_fp12* beta, const uint8_t* valid,
+                         size_t batch_size, size_t count, bte_fp12* messages_out, void* stream);
 
 /* decryption.rs:205-233 partial_decrypt for N parties at once: sigma_j = sum_i q_j^i ct1_i,
  * i = 1..B.  q = N canonical scalars (HOST); sigma_out = N G1 Jacobian (device). */

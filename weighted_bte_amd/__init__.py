@@ -57,6 +57,13 @@ _SIGNATURES = {
     "bte_btx_precompute": [_vp, _vp, _sz, _vp, _vp, _vp],
     "bte_btx_open": [_vp, _sz, _vp, _vp, _vp, _vp, _sz, _vp, _vp],
     "bte_btx_partial_decrypt": [_vp, _sz, _vp, _sz, _vp, _vp],
+    "bte_btx_precompute_batched": [_vp, _vp, _sz, _sz, _vp, _vp, _vp],
+    "bte_btx_open_batched": [_vp, _sz, _vp, _vp, _vp, _vp, _sz, _sz, _vp, _vp],
+    "bte_miller_loop_prepared_tiled": [_vp, _sz, _vp, _sz, _vp, _vp],
+    "bte_fft_g1_batched": [_vp, _sz, _sz, _i, _i, _vp],
+    "bte_fft_g2_batched": [_vp, _sz, _sz, _i, _i, _vp],
+    "bte_fft_cyclotomic_batched": [_vp, _sz, _sz, _i, _vp],
+    "bte_ifft_cyclotomic_prefix_unscaled_batched": [_vp, _sz, _sz, _sz, _vp],
     "bte_profile_enable": [_i],
     "bte_profile_read": [ctypes.c_char_p, _sz],
     "bte_probe_mac_rate": [ctypes.POINTER(ctypes.c_double), _vp],

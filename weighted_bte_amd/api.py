@@ -366,6 +366,29 @@ def btx_open(shares_jac, opening_keys, ct2, beta, valid):
     return out
 
 
+def btx_precompute_batched(ct1_affine, valid, kernel_lines, batch_size):
+    """precompute_batch for count = len(valid) / B batches fused in each launch."""
+    ct1 = _dev(ct1_affine)
+    n = ct1.numel() // G1_AFFINE
+    count = n // batch_size
+    v = (ctypes.c_uint8 * n)(*[1 if x else 0 for x in valid])
+    beta = _empty(n * FP12, ct1)
+    call("bte_btx_precompute_batched", _ptr(ct1), v, batch_size, count, _ptr(_dev(kernel_lines)),
+         _ptr(beta), _stream())
+    return beta
+
+
+def btx_open_batched(shares_jac, n_accepted, opening_keys, ct2, beta, valid, batch_size):
+    ct2 = _dev(ct2)
+    n = ct2.numel() // FP12
+    count = n // batch_size
+    v = (ctypes.c_uint8 * n)(*[1 if x else 0 for x in valid])
+    out = _empty(n * FP12, ct2)
+    call("bte_btx_open_batched", _ptr(_dev(shares_jac)), n_accepted, _ptr(_dev(opening_keys)), _ptr(ct2),
+         _ptr(_dev(beta)), v, batch_size, count, _ptr(out), _stream())
+    return out
+
+
 def btx_partial_decrypt(ct1_affine, q_scalars):
     """partial_decrypt (decryption.rs:205-233) for every party key q_j -> sigma (N x G1 Jacobian)."""
     ct1 = _dev(ct1_affine)

@@ -24,6 +24,11 @@ int bte_g1_batch_normalize(const bte_p1* in, size_t n, bte_p1_affine* out, void*
 int bte_g2_batch_normalize(const bte_p2* in, size_t n, bte_p2_affine* out, void* stream);
 int bte_g2_prepare_lines(const bte_p2_affine* q, size_t n, bte_fp6* lines, void* stream);
 int bte_miller_loop_prepared(const bte_fp6* lines, const bte_p1_affine* p, size_t n, bte_fp12* out, void* stream);
+int bte_miller_loop_prepared_tiled(const bte_fp6* lines, size_t ntables, const bte_p1_affine* p, size_t n,
+                                   bte_fp12* out, void* stream);
+int bte_fft_g1_batched(bte_p1* values, size_t n, size_t count, int inverse, int scaled, void* stream);
+int bte_ifft_cyclotomic_prefix_unscaled_batched(bte_fp12* values, size_t n, size_t count, size_t output_count,
+                                                void* stream);
 int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine* q, size_t npairs, const uint64_t* seg_offsets,
                           size_t nseg, bte_fp12* out, void* stream);
 int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
@@ -132,10 +137,29 @@ __global__ void __launch_bounds__(64) k_scale_all_g2(g2j* __restrict__ v, size_t
   if (i < n) v[i] = group_scalar_mul(v[i], s);
 }
 
-__global__ void k_g1_pad(const g1a* __restrict__ in, size_t B, size_t m, g1j* __restrict__ out) {
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  out[i] = i < B ? jac_from_aff(in[i]) : jac_identity<fp>();
+// count batches: out[b m + i] = in[b B + i] (i < B) else the identity.
+__global__ void k_g1_pad(const g1a* __restrict__ in, size_t B, size_t m, size_t count, g1j* __restrict__ out) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= m * count) return;
+  const size_t b = g / m, i = g % m;
+  out[g] = i < B ? jac_from_aff(in[b * B + i]) : jac_identity<fp>();
+}
+
+// out[b B + k] = in[b m + k] for k < B.
+__global__ void k_gather_prefix(const fp12* __restrict__ in, size_t m, size_t B, size_t count, fp12* __restrict__ out) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= B * count) return;
+  out[g] = in[(g / B) * m + g % B];
+}
+
+// Pairs of batch b, slot i, party p: P = shares[b T + p], Q = keys[i T + p].
+__global__ void k_open_pairs(const g1a* __restrict__ shares, const g2a* __restrict__ keys, size_t T, size_t B,
+                             size_t count, g1a* __restrict__ P, g2a* __restrict__ Q) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= T * B * count) return;
+  const size_t p = g % T, i = (g / T) % B, b = g / (T * B);
+  P[g] = shares[b * T + p];
+  Q[g] = keys[i * T + p];
 }
 
 template <class T> __global__ void k_tile(const T* __restrict__ in, size_t n, size_t reps, T* __restrict__ out) {
@@ -321,40 +345,46 @@ extern "C" int bte_btx_prepare(const bte_p2_affine* material, size_t total_weigh
   return BTE_OK;
 }
 
-// decryption.rs:555-603 precompute_batch.  ct1: B G1 affine first components (identity for
-// invalid slots), valid: B flags (host).  beta_out: B Fp12 on the device.
-extern "C" int bte_btx_precompute(const bte_p1_affine* ct1, const uint8_t* valid, size_t batch_size,
-                                  const bte_fp6* kernel_lines, bte_fp12* beta_out, void* stream) {
+// decryption.rs:555-603 precompute_batch for `count` batches of B ciphertexts against one
+// prepared kernel (same committee and batch size).  ct1: count x B G1 affine (identity in
+// invalid slots), valid: count x B flags (host).  beta_out: count x B Fp12 on the device.
+extern "C" int bte_btx_precompute_batched(const bte_p1_affine* ct1, const uint8_t* valid, size_t batch_size,
+                                          size_t count, const bte_fp6* kernel_lines, bte_fp12* beta_out,
+                                          void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  const size_t B = batch_size;
+  const size_t B = batch_size, K = count;
   if (!ct1 || !valid || !kernel_lines || !beta_out) return set_error(BTE_EINVAL, "btx_precompute: null pointer");
   if (B == 0) return set_error(BTE_EINVAL, "btx_precompute: InvalidBatchSize(0)");
+  if (K == 0) return BTE_OK;
   const size_t m = next_pow2(2 * B);
   Scratch sc(s);
   g1j* c;
   g1a* ca;
-  fp12 *ml, *conv;
+  fp12 *ml, *conv, *pref;
   uint8_t* d_valid;
-  BTE_TRY(sc.get(&c, m));
-  BTE_TRY(sc.get(&ca, m));
-  BTE_TRY(sc.get(&ml, m));
-  BTE_TRY(sc.get(&conv, m));
-  BTE_TRY(sc.upload(&d_valid, valid, B));
+  BTE_TRY(sc.get(&c, m * K));
+  BTE_TRY(sc.get(&ca, m * K));
+  BTE_TRY(sc.get(&ml, m * K));
+  BTE_TRY(sc.get(&conv, m * K));
+  BTE_TRY(sc.get(&pref, B * K));
+  BTE_TRY(sc.upload(&d_valid, valid, B * K));
   BTE_TRY(sc.uploads_done());
   PhaseTimer pt(s);
   pt.begin();
-  hipLaunchKernelGGL(k_g1_pad, dim3(blocks_for(m, 256)), dim3(256), 0, s, (const g1a*)ct1, B, m, c);
-  BTE_TRY(bte_fft_g1((bte_p1*)c, m, 0, 0, s));
-  BTE_TRY(bte_g1_batch_normalize((const bte_p1*)c, m, (bte_p1_affine*)ca, s));
+  hipLaunchKernelGGL(k_g1_pad, dim3(blocks_for(m * K, 256)), dim3(256), 0, s, (const g1a*)ct1, B, m, K, c);
+  BTE_TRY(bte_fft_g1_batched((bte_p1*)c, m, K, 0, 0, s));
+  BTE_TRY(bte_g1_batch_normalize((const bte_p1*)c, m * K, (bte_p1_affine*)ca, s));
   pt.mark("g1_fft");
-  BTE_TRY(bte_miller_loop_prepared(kernel_lines, (const bte_p1_affine*)ca, m, (bte_fp12*)ml, s));
+  BTE_TRY(bte_miller_loop_prepared_tiled(kernel_lines, m, (const bte_p1_affine*)ca, m * K, (bte_fp12*)ml, s));
   pt.mark("miller");
-  BTE_TRY(bte_final_exp_easy((const bte_fp12*)ml, m, (bte_fp12*)conv, s));
+  BTE_TRY(bte_final_exp_easy((const bte_fp12*)ml, m * K, (bte_fp12*)conv, s));
   pt.mark("easy");
-  BTE_TRY(bte_ifft_cyclotomic_prefix_unscaled((bte_fp12*)conv, m, B, s));
+  BTE_TRY(bte_ifft_cyclotomic_prefix_unscaled_batched((bte_fp12*)conv, m, K, B, s));
+  hipLaunchKernelGGL(k_gather_prefix, dim3(blocks_for(B * K, 256)), dim3(256), 0, s, (const fp12*)conv, m, B, K, pref);
   pt.mark("cyclo_ifft");
-  BTE_TRY(bte_final_exp_hard((const bte_fp12*)conv, B, beta_out, s));
-  hipLaunchKernelGGL(k_mask_invalid, dim3(blocks_for(B, 256)), dim3(256), 0, s, (fp12*)beta_out, (const uint8_t*)d_valid, B);
+  BTE_TRY(bte_final_exp_hard((const bte_fp12*)pref, B * K, beta_out, s));
+  hipLaunchKernelGGL(k_mask_invalid, dim3(blocks_for(B * K, 256)), dim3(256), 0, s, (fp12*)beta_out,
+                     (const uint8_t*)d_valid, B * K);
   pt.mark("hard");
   BTE_CHECK_LAUNCH();
   if (pt.on) {
@@ -364,41 +394,53 @@ extern "C" int bte_btx_precompute(const bte_p1_affine* ct1, const uint8_t* valid
   return BTE_OK;
 }
 
-// decryption.rs:607-695 open_batch.  shares: T G1 Jacobian (committee order), opening_keys from
-// bte_btx_prepare, ct2 / beta: B Fp12, valid: B flags (host).  messages_out: B Fp12; invalid
-// slots (the reference's None) are written as one.
-extern "C" int bte_btx_open(const bte_p1* shares, size_t n_accepted, const bte_p2_affine* opening_keys,
-                            const bte_fp12* ct2, const bte_fp12* beta, const uint8_t* valid, size_t batch_size,
-                            bte_fp12* messages_out, void* stream) {
+extern "C" int bte_btx_precompute(const bte_p1_affine* ct1, const uint8_t* valid, size_t batch_size,
+                                  const bte_fp6* kernel_lines, bte_fp12* beta_out, void* stream) {
+  return bte_btx_precompute_batched(ct1, valid, batch_size, 1, kernel_lines, beta_out, stream);
+}
+
+// decryption.rs:607-695 open_batch for `count` batches of one committee.  shares: count x T G1
+// Jacobian (committee order per batch), opening_keys from bte_btx_prepare, ct2 / beta: count x B
+// Fp12, valid: count x B flags (host).  messages_out: count x B Fp12; invalid slots (the
+// reference's None) are written as one.
+extern "C" int bte_btx_open_batched(const bte_p1* shares, size_t n_accepted, const bte_p2_affine* opening_keys,
+                                    const bte_fp12* ct2, const bte_fp12* beta, const uint8_t* valid,
+                                    size_t batch_size, size_t count, bte_fp12* messages_out, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  const size_t B = batch_size, T = n_accepted;
+  const size_t B = batch_size, T = n_accepted, K = count;
   if (!shares || !opening_keys || !ct2 || !beta || !valid || !messages_out)
     return set_error(BTE_EINVAL, "btx_open: null pointer");
   if (B == 0 || T == 0) return set_error(BTE_EINVAL, "btx_open: empty batch or committee");
+  if (K == 0) return BTE_OK;
+  const size_t npairs = K * B * T;
   Scratch sc(s);
-  g1a *sa, *tiled;
+  g1a *sa, *P;
+  g2a* Q;
   fp12 *ml, *alpha;
   uint64_t* d_off;
   uint8_t* d_valid;
-  std::vector<uint64_t> off(B + 1);
-  for (size_t i = 0; i <= B; ++i) off[i] = i * T;
-  BTE_TRY(sc.get(&sa, T));
-  BTE_TRY(sc.get(&tiled, B * T));
-  BTE_TRY(sc.get(&ml, B));
-  BTE_TRY(sc.get(&alpha, B));
-  BTE_TRY(sc.upload(&d_off, off.data(), B + 1));
-  BTE_TRY(sc.upload(&d_valid, valid, B));
+  std::vector<uint64_t> off(K * B + 1);
+  for (size_t i = 0; i <= K * B; ++i) off[i] = i * T;
+  BTE_TRY(sc.get(&sa, K * T));
+  BTE_TRY(sc.get(&P, npairs));
+  BTE_TRY(sc.get(&Q, npairs));
+  BTE_TRY(sc.get(&ml, K * B));
+  BTE_TRY(sc.get(&alpha, K * B));
+  BTE_TRY(sc.upload(&d_off, off.data(), K * B + 1));
+  BTE_TRY(sc.upload(&d_valid, valid, K * B));
   BTE_TRY(sc.uploads_done());
   PhaseTimer pt(s);
   pt.begin();
-  BTE_TRY(bte_g1_batch_normalize(shares, T, (bte_p1_affine*)sa, s));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile<g1a>), dim3(blocks_for(B * T, 256)), dim3(256), 0, s, (const g1a*)sa, T, B, tiled);
-  BTE_TRY(bte_multi_miller_loop((const bte_p1_affine*)tiled, opening_keys, B * T, d_off, B, (bte_fp12*)ml, s));
+  BTE_TRY(bte_g1_batch_normalize(shares, K * T, (bte_p1_affine*)sa, s));
+  hipLaunchKernelGGL(k_open_pairs, dim3(blocks_for(npairs, 256)), dim3(256), 0, s, (const g1a*)sa,
+                     (const g2a*)opening_keys, T, B, K, P, Q);
+  BTE_TRY(bte_multi_miller_loop((const bte_p1_affine*)P, (const bte_p2_affine*)Q, npairs, d_off, K * B,
+                                (bte_fp12*)ml, s));
   pt.mark("multi_miller");
-  BTE_TRY(bte_final_exp((const bte_fp12*)ml, B, (bte_fp12*)alpha, s));
+  BTE_TRY(bte_final_exp((const bte_fp12*)ml, K * B, (bte_fp12*)alpha, s));
   pt.mark("final_exp");
-  hipLaunchKernelGGL(k_open_combine, dim3(blocks_for(B, 64)), dim3(64), 0, s, (const fp12*)ct2, (const fp12*)alpha,
-                     (const fp12*)beta, (const uint8_t*)d_valid, B, (fp12*)messages_out);
+  hipLaunchKernelGGL(k_open_combine, dim3(blocks_for(K * B, 64)), dim3(64), 0, s, (const fp12*)ct2,
+                     (const fp12*)alpha, (const fp12*)beta, (const uint8_t*)d_valid, K * B, (fp12*)messages_out);
   pt.mark("combine");
   BTE_CHECK_LAUNCH();
   if (pt.on) {
@@ -406,6 +448,13 @@ extern "C" int bte_btx_open(const bte_p1* shares, size_t n_accepted, const bte_p
     pt.publish();
   }
   return BTE_OK;
+}
+
+extern "C" int bte_btx_open(const bte_p1* shares, size_t n_accepted, const bte_p2_affine* opening_keys,
+                            const bte_fp12* ct2, const bte_fp12* beta, const uint8_t* valid, size_t batch_size,
+                            bte_fp12* messages_out, void* stream) {
+  return bte_btx_open_batched(shares, n_accepted, opening_keys, ct2, beta, valid, batch_size, 1, messages_out,
+                              stream);
 }
 
 // decryption.rs:205-233 partial_decrypt for every party at once: sigma_j = sum_i q_j^i C_i

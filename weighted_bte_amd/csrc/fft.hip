@@ -140,24 +140,28 @@ BTE_DEV size_t bitrev(size_t i, int bits) {
   return bits ? (size_t)(__builtin_bitreverse64((uint64_t)i) >> (64 - bits)) : 0;
 }
 
-// fft.rs:621-632
-template <class T> __global__ void k_bit_reverse(T* v, size_t n, int logn) {
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// fft.rs:621-632, over `count` transforms of size n stored back to back.
+template <class T> __global__ void k_bit_reverse(T* v, size_t n, int logn, size_t count) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= n * count) return;
+  T* b = v + (g >> logn) * n;
+  const size_t i = g & (n - 1);
   const size_t r = bitrev(i, logn);
   if (i < r) {
-    T a = v[i];
-    v[i] = v[r];
-    v[r] = a;
+    T a = b[i];
+    b[i] = b[r];
+    b[r] = a;
   }
 }
 
 // One DIT stage for butterfly size 2 * 2^log_half; twiddle index k << (logn - 1 - log_half).
 template <class F>
 __global__ void __launch_bounds__(64) k_group_stage(jac<F>* __restrict__ v, size_t n, int logn, int log_half,
-                                                    const uint32_t* __restrict__ tw) {
-  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (t >= n / 2) return;
+                                                    const uint32_t* __restrict__ tw, size_t count) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= (n / 2) * count) return;
+  v += (g >> (logn - 1)) * n;
+  const size_t t = g & (n / 2 - 1);
   const size_t half = (size_t)1 << log_half;
   const size_t k = t & (half - 1);
   const size_t i = ((t >> log_half) << (log_half + 1)) + k;
@@ -176,9 +180,11 @@ __global__ void __launch_bounds__(64) k_group_scale(jac<F>* __restrict__ v, size
 }
 
 __global__ void __launch_bounds__(64) k_cyc_stage(fp12* __restrict__ v, size_t n, int logn, int log_half,
-                                                  const cyc_plan* __restrict__ plans) {
-  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (t >= n / 2) return;
+                                                  const cyc_plan* __restrict__ plans, size_t count) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= (n / 2) * count) return;
+  v += (g >> (logn - 1)) * n;
+  const size_t t = g & (n / 2 - 1);
   const size_t half = (size_t)1 << log_half;
   const size_t k = t & (half - 1);
   const size_t i = ((t >> log_half) << (log_half + 1)) + k;
@@ -191,9 +197,11 @@ __global__ void __launch_bounds__(64) k_cyc_stage(fp12* __restrict__ v, size_t n
 
 // Last stage of the prefix inverse transform (fft.rs:322-349): only outputs k < count.
 __global__ void __launch_bounds__(64) k_cyc_prefix_last(fp12* __restrict__ v, size_t n, size_t count,
-                                                        const cyc_plan* __restrict__ plans) {
-  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (k >= count) return;
+                                                        const cyc_plan* __restrict__ plans, size_t ntrans) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= count * ntrans) return;
+  v += (g / count) * n;
+  const size_t k = g % count;
   const size_t half = n / 2;
   fp12 w = v[half + k];
   if (k) w = cyclo_pow_plan(w, plans + k);
@@ -221,9 +229,11 @@ __global__ void __launch_bounds__(64) k_cyc_pow_plans(fp12* __restrict__ v, size
 // ---------------------------------------------------------------------------------------------
 // Drivers
 // ---------------------------------------------------------------------------------------------
-template <class F> static int group_fft(jac<F>* v, size_t n, bool inverse, bool scaled, hipStream_t s) {
+template <class F>
+static int group_fft(jac<F>* v, size_t n, size_t count, bool inverse, bool scaled, hipStream_t s) {
   const int logn = log2_exact(n);
   if (logn < 0) return set_error(BTE_EINVAL, "fft: InvalidBatchSize(%zu)", n);
+  if (count == 0) return BTE_OK;
   if (!v) return set_error(BTE_EINVAL, "fft: null values");
   if (n == 1) return BTE_OK;  // a one-point transform is the identity map (n^-1 = 1)
   DomainTables* t;
@@ -231,14 +241,15 @@ template <class F> static int group_fft(jac<F>* v, size_t n, bool inverse, bool 
   if (rc) return rc;
   PhaseTimer pt(s);
   pt.begin();
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse<jac<F>>), dim3(blocks_for(n, 256)), dim3(256), 0, s, v, n, logn);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse<jac<F>>), dim3(blocks_for(n * count, 256)), dim3(256), 0, s, v, n,
+                     logn, count);
   for (int lh = 0; lh < logn; ++lh)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage<F>), dim3(blocks_for(n / 2, 64)), dim3(64), 0, s, v, n, logn,
-                       lh, (const uint32_t*)t->twiddles);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage<F>), dim3(blocks_for(n / 2 * count, 64)), dim3(64), 0, s, v, n,
+                       logn, lh, (const uint32_t*)t->twiddles, count);
   pt.mark("stages");
   if (inverse && scaled)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_scale<F>), dim3(blocks_for(n, 64)), dim3(64), 0, s, v, n,
-                       (const uint32_t*)t->size_inv);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_scale<F>), dim3(blocks_for(n * count, 64)), dim3(64), 0, s, v,
+                       n * count, (const uint32_t*)t->size_inv);
   pt.mark("scale");
   BTE_CHECK_LAUNCH();
   if (pt.on) {
@@ -248,26 +259,29 @@ template <class F> static int group_fft(jac<F>* v, size_t n, bool inverse, bool 
   return BTE_OK;
 }
 
-// mode 0: full transform; mode 1: prefix inverse (last stage only for `count` outputs)
-static int cyclotomic_fft(fp12* v, size_t n, bool inverse, bool scaled, size_t count, int mode, hipStream_t s) {
+// mode 0: full transform; mode 1: prefix inverse (last stage only for `out_count` outputs)
+static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool scaled, size_t out_count, int mode,
+                          hipStream_t s) {
   const int logn = log2_exact(n);
   if (logn < 0) return set_error(BTE_EINVAL, "fft: InvalidBatchSize(%zu)", n);
+  if (count == 0) return BTE_OK;
   if (!v) return set_error(BTE_EINVAL, "fft: null values");
-  if (mode == 1 && count > n / 2) return set_error(BTE_EINVAL, "prefix fft: output_count > n/2");
+  if (mode == 1 && out_count > n / 2) return set_error(BTE_EINVAL, "prefix fft: output_count > n/2");
   if (n == 1) return BTE_OK;
   DomainTables* t;
   int rc = domain_tables(logn, inverse, true, &t, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse<fp12>), dim3(blocks_for(n, 256)), dim3(256), 0, s, v, n, logn);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse<fp12>), dim3(blocks_for(n * count, 256)), dim3(256), 0, s, v, n,
+                     logn, count);
   const int full_stages = mode == 1 ? logn - 1 : logn;
   for (int lh = 0; lh < full_stages; ++lh)
-    hipLaunchKernelGGL(k_cyc_stage, dim3(blocks_for(n / 2, 64)), dim3(64), 0, s, v, n, logn, lh,
-                       (const cyc_plan*)t->plans);
-  if (mode == 1 && count)
-    hipLaunchKernelGGL(k_cyc_prefix_last, dim3(blocks_for(count, 64)), dim3(64), 0, s, v, n, count,
-                       (const cyc_plan*)t->plans);
+    hipLaunchKernelGGL(k_cyc_stage, dim3(blocks_for(n / 2 * count, 64)), dim3(64), 0, s, v, n, logn, lh,
+                       (const cyc_plan*)t->plans, count);
+  if (mode == 1 && out_count)
+    hipLaunchKernelGGL(k_cyc_prefix_last, dim3(blocks_for(out_count * count, 64)), dim3(64), 0, s, v, n, out_count,
+                       (const cyc_plan*)t->plans, count);
   if (mode == 0 && inverse && scaled)
-    hipLaunchKernelGGL(k_cyc_scale, dim3(blocks_for(n, 64)), dim3(64), 0, s, v, n,
+    hipLaunchKernelGGL(k_cyc_scale, dim3(blocks_for(n * count, 64)), dim3(64), 0, s, v, n * count,
                        (const cyc_plan*)t->size_inv_plan);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
@@ -293,24 +307,43 @@ static int upload_plans(const uint8_t* host_scalars, size_t n, cyc_plan** out, h
 using namespace bte;
 
 extern "C" int bte_fft_g1(bte_p1* values, size_t n, int inverse, int scaled, void* stream) {
-  return group_fft<fp>((jac<fp>*)values, n, inverse != 0, scaled != 0, (hipStream_t)stream);
+  return group_fft<fp>((jac<fp>*)values, n, 1, inverse != 0, scaled != 0, (hipStream_t)stream);
 }
 
 extern "C" int bte_fft_g2(bte_p2* values, size_t n, int inverse, int scaled, void* stream) {
-  return group_fft<fp2>((jac<fp2>*)values, n, inverse != 0, scaled != 0, (hipStream_t)stream);
+  return group_fft<fp2>((jac<fp2>*)values, n, 1, inverse != 0, scaled != 0, (hipStream_t)stream);
 }
 
 extern "C" int bte_fft_gt(bte_fp12* values, size_t n, int inverse, int scaled, void* stream) {
-  return cyclotomic_fft((fp12*)values, n, inverse != 0, scaled != 0, 0, 0, (hipStream_t)stream);
+  return cyclotomic_fft((fp12*)values, n, 1, inverse != 0, scaled != 0, 0, 0, (hipStream_t)stream);
 }
 
 extern "C" int bte_fft_cyclotomic(bte_fp12* values, size_t n, int inverse, void* stream) {
-  return cyclotomic_fft((fp12*)values, n, inverse != 0, false, 0, 0, (hipStream_t)stream);
+  return cyclotomic_fft((fp12*)values, n, 1, inverse != 0, false, 0, 0, (hipStream_t)stream);
 }
 
 extern "C" int bte_ifft_cyclotomic_prefix_unscaled(bte_fp12* values, size_t n, size_t output_count,
                                                    void* stream) {
-  return cyclotomic_fft((fp12*)values, n, true, false, output_count, 1, (hipStream_t)stream);
+  return cyclotomic_fft((fp12*)values, n, 1, true, false, output_count, 1, (hipStream_t)stream);
+}
+
+// Batched forms: `count` independent transforms of size n stored back to back (one launch per
+// stage for all of them).
+extern "C" int bte_fft_g1_batched(bte_p1* values, size_t n, size_t count, int inverse, int scaled, void* stream) {
+  return group_fft<fp>((jac<fp>*)values, n, count, inverse != 0, scaled != 0, (hipStream_t)stream);
+}
+
+extern "C" int bte_fft_g2_batched(bte_p2* values, size_t n, size_t count, int inverse, int scaled, void* stream) {
+  return group_fft<fp2>((jac<fp2>*)values, n, count, inverse != 0, scaled != 0, (hipStream_t)stream);
+}
+
+extern "C" int bte_fft_cyclotomic_batched(bte_fp12* values, size_t n, size_t count, int inverse, void* stream) {
+  return cyclotomic_fft((fp12*)values, n, count, inverse != 0, false, 0, 0, (hipStream_t)stream);
+}
+
+extern "C" int bte_ifft_cyclotomic_prefix_unscaled_batched(bte_fp12* values, size_t n, size_t count,
+                                                           size_t output_count, void* stream) {
+  return cyclotomic_fft((fp12*)values, n, count, true, false, output_count, 1, (hipStream_t)stream);
 }
 
 extern "C" int bte_cyclotomic_pow_scalars(bte_fp12* values, const uint8_t* host_scalars, size_t n,

@@ -113,11 +113,11 @@ __global__ void __launch_bounds__(64) k_prepare_lines(const g2a* __restrict__ q,
   }
 }
 
-__global__ void __launch_bounds__(64) k_miller_prepared(const line_t* __restrict__ lines, const g1a* __restrict__ p,
-                                                        size_t n, fp12* __restrict__ out) {
+__global__ void __launch_bounds__(64) k_miller_prepared(const line_t* __restrict__ lines, size_t ntables,
+                                                        const g1a* __restrict__ p, size_t n, fp12* __restrict__ out) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const line_t* L = lines + i * MILLER_LINES;
+  const line_t* L = lines + (i % ntables) * MILLER_LINES;
   const g1a P = p[i];
   if (aff_is_identity(P) || fp2_is_zero(L[0].cy)) {
     out[i] = fp12_one();
@@ -213,7 +213,18 @@ extern "C" int bte_miller_loop_prepared(const bte_fp6* lines, const bte_p1_affin
   if (n == 0) return BTE_OK;
   if (!lines || !p || !out) return set_error(BTE_EINVAL, "miller_loop_prepared: null pointer");
   hipLaunchKernelGGL(k_miller_prepared, dim3(blocks_for(n, 64)), dim3(64), 0, (hipStream_t)stream,
-                     (const line_t*)lines, (const g1a*)p, n, (fp12*)out);
+                     (const line_t*)lines, n, (const g1a*)p, n, (fp12*)out);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+// Pair i uses line table i % ntables (several batches against one prepared kernel).
+extern "C" int bte_miller_loop_prepared_tiled(const bte_fp6* lines, size_t ntables, const bte_p1_affine* p, size_t n,
+                                              bte_fp12* out, void* stream) {
+  if (n == 0) return BTE_OK;
+  if (!lines || !p || !out || ntables == 0) return set_error(BTE_EINVAL, "miller_loop_prepared_tiled: bad arguments");
+  hipLaunchKernelGGL(k_miller_prepared, dim3(blocks_for(n, 64)), dim3(64), 0, (hipStream_t)stream,
+                     (const line_t*)lines, ntables, (const g1a*)p, n, (fp12*)out);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
