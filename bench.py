@@ -70,6 +70,104 @@ def cpu_baseline(logn: int):
     return ocpu.time_g1_msm_sample(lib, logn)
 
 
+def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batches=2, with_cpu=True):
+    """weighted_btx batch decryption at B = 512 with n = 128 Pareto(1.16) weights scaled to
+    W = 4096 (threshold W/2, heaviest-first committee).  Reports ctxts/s for the plain per-call
+    path (one batch per call, timed over `plain_batches` batches) and the fused path (`batches`
+    batches per launch), cold committee (prepare + precompute + open) and cached committee
+    (precompute + open), and checks every plaintext against gT^s computed separately."""
+    import torch
+
+    import weighted_bte_amd as W
+    from weighted_bte_amd import api, workloads
+
+    t0 = time.time()
+    weights = workloads.pareto_weights(n_parties, total_weight)
+    wl = workloads.BtxWorkload(weights, total_weight // 2, B, seed=5)
+    committee = workloads.heaviest_first(weights, wl.t)
+    T = len(committee)
+    ct1, ct2, msgs = wl.encrypt(B, batches)
+    valid = [True] * (B * batches)
+    torch.cuda.synchronize()
+    t_setup = time.time() - t0
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - a
+
+    # party side: every party's share for every batch (batched G1 MSMs, N segments of B terms)
+    def shares_for(nb):
+        sh = []
+        for b in range(nb):
+            sig = api.btx_partial_decrypt(ct1[b * B * 96:(b + 1) * B * 96], wl.q)
+            sh.append(torch.cat([sig[144 * j:144 * (j + 1)] for j in committee]))
+        return torch.cat(sh)
+    shares_for(1)  # warm-up (domain tables, workspaces)
+    _, t_partial1 = timed(lambda: shares_for(1))
+    shares = shares_for(batches)
+
+    prep = lambda: api.btx_prepare(wl.material, wl.W, wl.L, wl.offsets, committee, B)
+    prep()
+    (lines, keys), t_prepare = timed(prep)
+
+    def plain(nb):
+        outs = []
+        for b in range(nb):
+            sl = slice(b * B, (b + 1) * B)
+            beta = api.btx_precompute(ct1[b * B * 96:(b + 1) * B * 96], valid[sl], lines)
+            outs.append(api.btx_open(shares[b * T * 144:(b + 1) * T * 144], keys,
+                                     ct2[b * B * 576:(b + 1) * B * 576], beta, valid[sl]))
+        return torch.cat(outs)
+
+    def fused():
+        beta = api.btx_precompute_batched(ct1, valid, lines, B)
+        return api.btx_open_batched(shares, T, keys, ct2, beta, valid, B)
+
+    plain(1)
+    out_plain, t_plain = timed(lambda: plain(plain_batches))
+    fused()
+    out_fused, t_fused = timed(fused)
+    ok = bool(torch.equal(out_plain, msgs[:plain_batches * B * 576]) and torch.equal(out_fused, msgs))
+    # phase breakdown of one fused pass (HIP events on the library's stream)
+    W.profile_enable(True)
+    beta = api.btx_precompute_batched(ct1, valid, lines, B)
+    ph_pre = W.profile_read()
+    api.btx_open_batched(shares, T, keys, ct2, beta, valid, B)
+    ph_open = W.profile_read()
+    W.profile_enable(False)
+    n = B * batches
+    cpu = None
+    if with_cpu:
+        try:
+            from oracle import cpu as ocpu
+            lib = ocpu.load()
+            cpu = ocpu.time_btx_decrypt_sample(lib, B=B, T=T) if lib is not None else None
+        except ImportError:
+            cpu = None
+    fused_rate = n / t_fused
+    return {
+        "workload": "weighted_btx decrypt", "B": B, "fused_batches": batches, "plain_batches": plain_batches,
+        "parties": n_parties, "total_weight": total_weight, "committee": T, "plaintexts_bit_exact": ok,
+        "setup_s": round(t_setup, 2),
+        "prepare_ms": round(t_prepare * 1e3, 2),
+        "partial_decrypt_ms_per_batch_all_parties": round(t_partial1 * 1e3, 2),
+        "plain_ms_per_batch": round(t_plain * 1e3 / plain_batches, 2),
+        "fused_ms_total": round(t_fused * 1e3, 2),
+        "ctxts_per_s_cached_plain": round(plain_batches * B / t_plain, 1),
+        "ctxts_per_s_cached_fused": round(fused_rate, 1),
+        "ctxts_per_s_cold_plain": round(B / (t_prepare + t_plain / plain_batches), 1),
+        "ctxts_per_s_cold_fused": round(n / (t_prepare + t_fused), 1),
+        "fused_over_plain": round(fused_rate / (plain_batches * B / t_plain), 2),
+        "fused_over_cpu": round(fused_rate / cpu["value"], 2) if cpu else None,
+        "phases_ms_fused": {"precompute": {k: round(v, 2) for k, v in ph_pre.items()},
+                            "open": {k: round(v, 2) for k, v in ph_open.items()}},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +175,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--logn", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--decrypt-batches", type=int, default=64,
+                    help="batches of 512 ciphertexts in the fused decrypt measurement (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -136,6 +236,7 @@ def main():
     mac_peak = W.probe_mac_rate(torch.cuda.current_stream().cuda_stream)
 
     cpu = None if a.no_cpu else cpu_baseline(a.logn)
+    decrypt = bench_decrypt(a.decrypt_batches, with_cpu=not a.no_cpu) if a.decrypt_batches > 0 else None
 
     line = {
         "metric": "G1 MSM 2^%d ms (decrypted ctxts/s at B=512 reported separately)" % a.logn,
@@ -169,9 +270,10 @@ def main():
             "mixed_adds_per_launch": E,
         },
         "cpu_baseline": cpu,
+        "decrypt": decrypt,
     }
     print(json.dumps(line))
-    sys.exit(0 if parity else 1)
+    sys.exit(0 if parity and (decrypt is None or decrypt["plaintexts_bit_exact"]) else 1)
 
 
 if __name__ == "__main__":

@@ -87,3 +87,37 @@ def test_btx_config1_shape_with_invalid_slot(api):
             assert got[i] == O.F12_ONE
         else:
             assert got[i] == O.f12_pow(g, st.s[i]), "slot %d" % i
+
+
+def test_btx_fused_batches_equal_alone(api):
+    """Each result is the same alone (plain per-call path) and in the fused multi-batch path."""
+    from weighted_bte_amd import workloads
+
+    weights = [3, 1, 4, 1, 5]
+    wl = workloads.BtxWorkload(weights, threshold=7, max_batch=4, seed=9)
+    committee = workloads.heaviest_first(weights, wl.t)
+    T, B, K = len(committee), 4, 3
+    ct1, ct2, msgs = wl.encrypt(B, K)
+    valid = [True] * (B * K)
+    valid[5] = False  # batch 1, slot 1
+    raw = bytearray(api.to_host(ct1))
+    raw[96 * 5: 96 * 6] = bytes(96)
+    ct1 = api.to_device(bytes(raw))
+    lines, keys = api.btx_prepare(wl.material, wl.W, wl.L, wl.offsets, committee, B)
+    shares = []
+    for b in range(K):
+        sig = api.btx_partial_decrypt(ct1[b * B * 96:(b + 1) * B * 96], wl.q)
+        shares.append(b"".join(api.to_host(sig[144 * j:144 * (j + 1)]) for j in committee))
+    sh = api.to_device(b"".join(shares))
+    beta = api.btx_precompute_batched(ct1, valid, lines, B)
+    fused = api.to_host(api.btx_open_batched(sh, T, keys, ct2, beta, valid, B))
+    want = api.to_host(msgs)
+    for b in range(K):
+        sl = slice(b * B, (b + 1) * B)
+        be = api.btx_precompute(ct1[b * B * 96:(b + 1) * B * 96], valid[sl], lines)
+        alone = api.to_host(api.btx_open(sh[b * T * 144:(b + 1) * T * 144], keys,
+                                         ct2[b * B * 576:(b + 1) * B * 576], be, valid[sl]))
+        assert alone == fused[b * B * 576:(b + 1) * B * 576]
+    one = O.f12_to_bytes(O.F12_ONE)
+    for i in range(B * K):
+        assert fused[576 * i:576 * (i + 1)] == (want[576 * i:576 * (i + 1)] if valid[i] else one)

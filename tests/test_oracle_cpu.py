@@ -60,3 +60,27 @@ def test_c_consecutive_multiples_closed_form(clib):
     got = O.G1.to_affine(O.g1_jac_from_bytes(cpu.g1_msm(clib, pts, pack_scalars(sc))))
     want = O.G1.to_affine(O.G1.mul(O.g1_gen_jac(), sum((i + 1) * s for i, s in enumerate(sc)) % O.R))
     assert got == want
+
+
+def test_c_pairing_path_matches_python(clib):
+    import ctypes
+    from oracle import cpu
+    rnd = random.Random(21)
+    a, b = rnd.randrange(1, O.R), rnd.randrange(1, O.R)
+    P = O.G1.to_affine(O.G1.mul(O.g1_gen_jac(), a))
+    Q = O.G2.to_affine(O.G2.mul(O.g2_gen_jac(), b))
+    out = ctypes.create_string_buffer(576)
+    clib.oracle_miller_pairs(O.g1_affine_to_bytes(P), O.g2_affine_to_bytes(Q), 1, out)
+    fe = ctypes.create_string_buffer(576)
+    clib.oracle_final_exp(out.raw, 1, 1, 1, fe)
+    assert O.f12_from_bytes(fe.raw) == O.pairing(P, Q)
+    # easy alone is exact
+    f = tuple(tuple((rnd.randrange(O.P), rnd.randrange(O.P)) for _ in range(3)) for _ in range(2))
+    clib.oracle_final_exp(O.f12_to_bytes(f), 1, 1, 0, fe)
+    e = O.f12_from_bytes(fe.raw)
+    assert e == O.easy_final_exponentiation(f)
+    # decomposed power with a plan
+    s = rnd.randrange(O.R)
+    buf = ctypes.create_string_buffer(O.f12_to_bytes(e), 576)
+    clib.oracle_cyc_pow_plans(buf, cpu.plan_bytes(s), 1)
+    assert O.f12_from_bytes(buf.raw) == O.cyclotomic_mul_scalar_decomposed(e, s)
