@@ -194,6 +194,34 @@ int bte_btx_partial_decrypt(const bte_p1_affine* ct1, size_t batch_size, const u
                             size_t n_parties, bte_p1* sigma_out, void* stream);
 
 /* ---------------------------------------------------------------------------------------- *
+ * a16: weighted_pfe batch-decryption phases.  Key material (device) is one G2 affine array: the
+ * reference's d1 (B blocks of W points) followed by its d2 (B blocks of W).  alpha holds the B
+ * alpha points (canonical scalars, HOST).
+ * ---------------------------------------------------------------------------------------- */
+
+/* weighted_pfe/src/decryption.rs:421-537 prepare_decryption.  Outputs (device): d1_by_party
+ * (B x T G2 affine, slot-major, coefficients scaled by 1/2B), total_lines and mask_lines (B line
+ * tables each). */
+int bte_pfe_prepare(const bte_p2_affine* material, size_t total_weight, size_t batch_size,
+                    const uint64_t* party_offsets, size_t party_count, const uint32_t* accepted,
+                    size_t n_accepted, const uint8_t* alpha, const bte_p2_affine* global_key,
+                    bte_p2_affine* d1_by_party, bte_fp6* total_lines, bte_fp6* mask_lines,
+                    void* stream);
+/* decryption.rs:540-573 precompute_batch: first_cauchy (B G1 Jacobian) and pairing_cauchy
+ * (B Fp12) from ct1 (B G1 affine); B must be a power of two. */
+int bte_pfe_precompute(const bte_p1_affine* ct1, size_t batch_size, const bte_fp6* total_lines,
+                       bte_p1* first_cauchy, bte_fp12* pairing_cauchy, void* stream);
+/* decryption.rs:576-644 open_batch -> messages_out (B Fp12). */
+int bte_pfe_open(const bte_p1* shares, size_t n_accepted, const bte_p2_affine* d1_by_party,
+                 const bte_fp6* mask_lines, const bte_p1_affine* ct1, const bte_fp12* ct2,
+                 const bte_p1* first_cauchy, const bte_fp12* pairing_cauchy, const uint8_t* alpha,
+                 size_t batch_size, bte_fp12* messages_out, void* stream);
+/* decryption.rs:230-251 partial_decrypt for N parties: sigma_j = sum_i g_{j,i} ct1_i with the
+ * N x B fractions g (canonical scalars, device). */
+int bte_pfe_partial_decrypt(const bte_p1_affine* ct1, size_t batch_size, const uint8_t* fractions,
+                            size_t n_parties, bte_p1* sigma_out, void* stream);
+
+/* ---------------------------------------------------------------------------------------- *
  * Fixed-base generation of test and key material: out[i] = k_i * G for the standard
  * generators (affine).  Used to build bases k_i * G for MSM inputs and CRS powers.
  * ---------------------------------------------------------------------------------------- */

@@ -64,6 +64,10 @@ _SIGNATURES = {
     "bte_fft_g2_batched": [_vp, _sz, _sz, _i, _i, _vp],
     "bte_fft_cyclotomic_batched": [_vp, _sz, _sz, _i, _vp],
     "bte_ifft_cyclotomic_prefix_unscaled_batched": [_vp, _sz, _sz, _sz, _vp],
+    "bte_pfe_prepare": [_vp, _sz, _sz, _vp, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
+    "bte_pfe_precompute": [_vp, _sz, _vp, _vp, _vp, _vp],
+    "bte_pfe_open": [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp],
+    "bte_pfe_partial_decrypt": [_vp, _sz, _vp, _sz, _vp, _vp],
     "bte_profile_enable": [_i],
     "bte_profile_read": [ctypes.c_char_p, _sz],
     "bte_probe_mac_rate": [ctypes.POINTER(ctypes.c_double), _vp],

@@ -399,3 +399,56 @@ def btx_partial_decrypt(ct1_affine, q_scalars):
     out = _empty(N * G1_JAC, ct1)
     call("bte_btx_partial_decrypt", _ptr(ct1), B, buf, N, _ptr(out), _stream())
     return out
+
+
+# --- a16: weighted_pfe phases (weighted_pfe/src/decryption.rs) ------------------------------
+def _host_scalars(vals):
+    b = b"".join(int(v).to_bytes(32, "little") for v in vals)
+    return ctypes.create_string_buffer(b, len(b))
+
+
+def pfe_prepare(material, total_weight, batch_size, party_offsets, accepted, alpha, global_key):
+    """prepare_decryption (decryption.rs:421-537) -> (d1_by_party, total_lines, mask_lines)."""
+    material = _dev(material)
+    B, T = batch_size, len(accepted)
+    offs = (ctypes.c_uint64 * len(party_offsets))(*party_offsets)
+    acc = (ctypes.c_uint32 * T)(*accepted)
+    d1p = _empty(B * T * G2_AFFINE, material)
+    tl = _empty(B * LINES_BYTES, material)
+    ml = _empty(B * LINES_BYTES, material)
+    call("bte_pfe_prepare", _ptr(material), total_weight, B, offs, len(party_offsets) - 1, acc, T,
+         _host_scalars(alpha), _ptr(_dev(global_key)), _ptr(d1p), _ptr(tl), _ptr(ml), _stream())
+    return d1p, tl, ml
+
+
+def pfe_precompute(ct1_affine, total_lines):
+    """precompute_batch (decryption.rs:540-573) -> (first_cauchy, pairing_cauchy)."""
+    ct1 = _dev(ct1_affine)
+    B = ct1.numel() // G1_AFFINE
+    fc = _empty(B * G1_JAC, ct1)
+    pc = _empty(B * FP12, ct1)
+    call("bte_pfe_precompute", _ptr(ct1), B, _ptr(_dev(total_lines)), _ptr(fc), _ptr(pc), _stream())
+    return fc, pc
+
+
+def pfe_open(shares_jac, d1_by_party, mask_lines, ct1_affine, ct2, first_cauchy, pairing_cauchy, alpha):
+    """open_batch (decryption.rs:576-644) -> messages (B x Fp12)."""
+    shares = _dev(shares_jac)
+    ct2 = _dev(ct2)
+    B = ct2.numel() // FP12
+    out = _empty(B * FP12, ct2)
+    call("bte_pfe_open", _ptr(shares), shares.numel() // G1_JAC, _ptr(_dev(d1_by_party)),
+         _ptr(_dev(mask_lines)), _ptr(_dev(ct1_affine)), _ptr(ct2), _ptr(_dev(first_cauchy)),
+         _ptr(_dev(pairing_cauchy)), _host_scalars(alpha), B, _ptr(out), _stream())
+    return out
+
+
+def pfe_partial_decrypt(ct1_affine, fractions):
+    """partial_decrypt (decryption.rs:230-251); fractions[j] = [g_{j,0}, ..., g_{j,B-1}]."""
+    ct1 = _dev(ct1_affine)
+    B = ct1.numel() // G1_AFFINE
+    N = len(fractions)
+    fr = to_device(b"".join(int(g).to_bytes(32, "little") for row in fractions for g in row), ct1.device)
+    out = _empty(N * G1_JAC, ct1)
+    call("bte_pfe_partial_decrypt", _ptr(ct1), B, _ptr(fr), N, _ptr(out), _stream())
+    return out

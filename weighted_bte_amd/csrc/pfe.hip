@@ -1,0 +1,371 @@
+// weighted_pfe batch decryption on the GPU: the host control flow of
+// weighted_pfe/src/decryption.rs (prepare_decryption :421-537, precompute_batch :540-573,
+// open_batch :576-644, the Cauchy transforms cauchy_scaled_group :683-708 and
+// cauchy_scaled_cyclotomic :710-734, mul_group_signed_small :736-760) and partial_decrypt
+// (:230-251) over this library's kernels.  Protocol bookkeeping stays with the caller.
+//
+// Key material layout (device): one G2 affine array holding d1 (B blocks of W points,
+// d1[i W + w] = [Z(w) g_{owner(w), i}]_2) followed by d2 (B blocks of W,
+// d2[i W + w] = [Z(w) u_{owner(w), i}]_2), i.e. the reference's slot-major d1 then d2.
+#include <vector>
+
+#include "common.hpp"
+#include "curve.hpp"
+#include "cyclo.hpp"
+#include "host_field.hpp"
+
+extern "C" {
+int bte_fft_g1(bte_p1* values, size_t n, int inverse, int scaled, void* stream);
+int bte_fft_cyclotomic(bte_fp12* values, size_t n, int inverse, void* stream);
+int bte_cyclotomic_pow_small(bte_fp12* values, const int64_t* coefficients, size_t n, void* stream);
+int bte_cyclotomic_pow_scalars(bte_fp12* values, const uint8_t* host_scalars, size_t n, void* stream);
+int bte_g1_batch_normalize(const bte_p1* in, size_t n, bte_p1_affine* out, void* stream);
+int bte_g2_batch_normalize(const bte_p2* in, size_t n, bte_p2_affine* out, void* stream);
+int bte_g2_prepare_lines(const bte_p2_affine* q, size_t n, bte_fp6* lines, void* stream);
+int bte_miller_loop_prepared(const bte_fp6* lines, const bte_p1_affine* p, size_t n, bte_fp12* out, void* stream);
+int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine* q, size_t npairs, const uint64_t* seg_offsets,
+                          size_t nseg, bte_fp12* out, void* stream);
+int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
+int bte_final_exp_hard(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
+int bte_lagrange_at_zero(size_t total_weight, const uint64_t* selected, size_t n, uint8_t* out);
+}
+
+namespace bte {
+
+#define PFE_TRY(x)        \
+  do {                    \
+    int rc_ = (x);        \
+    if (rc_) return rc_;  \
+  } while (0)
+
+// v * c for a small signed integer by double-and-add (decryption.rs:736-760).
+template <class F> BTE_DEV jac<F> mul_signed_small(const jac<F>& v, int64_t c) {
+  if (c == 0) return jac_identity<F>();
+  uint64_t mag = c < 0 ? (uint64_t)(-c) : (uint64_t)c;
+  jac<F> base = v, result = jac_identity<F>();
+  while (mag) {
+    if (mag & 1) result = jac_add(result, base);
+    mag >>= 1;
+    if (mag) base = jac_dbl(base);
+  }
+  return c < 0 ? jac_neg(result) : result;
+}
+
+__global__ void __launch_bounds__(64) k_g1_mul_small(g1j* __restrict__ v, const int64_t* __restrict__ c, size_t n) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) v[i] = mul_signed_small(v[i], c[i]);
+}
+
+__global__ void __launch_bounds__(64) k_g1_mul_small_const(g1j* __restrict__ v, int64_t c, size_t n) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) v[i] = mul_signed_small(v[i], c);
+}
+
+// out[i] = -in[(i - 1) mod n]  (decryption.rs:694-701)
+__global__ void k_g1_shift_neg(const g1j* __restrict__ in, size_t n, g1j* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = jac_neg(in[(i + n - 1) % n]);
+}
+
+// out[i] = conj(in[(i - 1) mod n])  (decryption.rs:720-727)
+__global__ void k_cyc_shift_conj(const fp12* __restrict__ in, size_t n, fp12* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = fp12_conj(in[(i + n - 1) % n]);
+}
+
+template <class F> __global__ void k_jac_from_aff(const aff<F>* __restrict__ in, size_t n, jac<F>* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = jac_from_aff(in[i]);
+}
+
+// out[s] = sum of in[s * len .. (s+1) * len)
+__global__ void __launch_bounds__(64) k_g2_sum_runs(const g2j* __restrict__ in, size_t len, size_t nout,
+                                                    g2j* __restrict__ out) {
+  __shared__ g2j sh[64];
+  const size_t s = blockIdx.x;
+  if (s >= nout) return;
+  const int tid = threadIdx.x;
+  g2j acc = jac_identity<fp2>();
+  for (size_t i = tid; i < len; i += 64) acc = jac_add(acc, in[s * len + i]);
+  sh[tid] = acc;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (tid < w) sh[tid] = jac_add(sh[tid], sh[tid + w]);
+    __syncthreads();
+  }
+  if (tid == 0) out[s] = sh[0];
+}
+
+// mask[i] = d2[i] * (a_i (a_i + 1)) - V * a_i  (decryption.rs:492-501); scalars canonical 8 x u32.
+__global__ void __launch_bounds__(64) k_pfe_mask(const g2j* __restrict__ d2, const g2a* __restrict__ V,
+                                                 const uint32_t* __restrict__ aa1, const uint32_t* __restrict__ a,
+                                                 size_t n, g2j* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  g2j x = group_scalar_mul(d2[i], aa1 + 8 * i);
+  g2j y = group_scalar_mul(jac_from_aff(*V), a + 8 * i);
+  out[i] = jac_sub(x, y);
+}
+
+// left[slot T + p] = shares2B[p] - cauchy[slot]  (decryption.rs:597-605)
+__global__ void k_pfe_left(const g1j* __restrict__ shares2b, const g1j* __restrict__ cauchy, size_t T, size_t B,
+                           g1j* __restrict__ out) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= T * B) return;
+  out[g] = jac_sub(shares2b[g % T], cauchy[g / T]);
+}
+
+// first[i] * cauchy[i] (cyclotomic products before the decomposed power)
+__global__ void __launch_bounds__(64) k_fp12_mul_pairs(const fp12* __restrict__ a, const fp12* __restrict__ b, size_t n,
+                                                       fp12* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = fp12_mul(a[i], b[i]);
+}
+
+// combined = mask_pairing * conj(subtracted); message = ct2 * conj(hard(combined)) is finished
+// by the caller-side kernels below.
+__global__ void __launch_bounds__(64) k_pfe_combine(const fp12* __restrict__ mask_pair, const fp12* __restrict__ sub,
+                                                    size_t n, fp12* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = fp12_mul(mask_pair[i], fp12_conj(sub[i]));
+}
+
+__global__ void __launch_bounds__(64) k_pfe_unmask(const fp12* __restrict__ ct2, const fp12* __restrict__ mask, size_t n,
+                                                   fp12* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = fp12_mul(ct2[i], fp12_conj(mask[i]));
+}
+
+struct PScratch {
+  hipStream_t s;
+  std::vector<void*> ptrs;
+  explicit PScratch(hipStream_t st) : s(st) {}
+  ~PScratch() {
+    for (void* p : ptrs) (void)hipFreeAsync(p, s);
+  }
+  template <class T> int get(T** p, size_t count) {
+    hipError_t e = hipMallocAsync((void**)p, (count ? count : 1) * sizeof(T), s);
+    if (e != hipSuccess) return set_error(BTE_ENOMEM, "scratch: %s", hipGetErrorString(e));
+    ptrs.push_back((void*)*p);
+    return BTE_OK;
+  }
+  template <class T> int upload(T** p, const T* host, size_t count) {
+    PFE_TRY(get(p, count));
+    BTE_HIP(hipMemcpyAsync(*p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return BTE_OK;
+  }
+  int done() {
+    BTE_HIP(hipStreamSynchronize(s));
+    return BTE_OK;
+  }
+};
+
+}  // namespace bte
+
+using namespace bte;
+
+// decryption.rs:421-537 prepare_decryption.  material = d1 then d2 (B x W each, device);
+// alpha: B canonical scalars (host); global_key: V (G2 affine, device).  Outputs (device):
+// d1_by_party (B x T G2 affine, slot-major, scaled by 1/(2B)), total_lines (B tables, the
+// per-slot sums), mask_lines (B tables).
+extern "C" int bte_pfe_prepare(const bte_p2_affine* material, size_t total_weight, size_t batch_size,
+                               const uint64_t* party_offsets, size_t party_count, const uint32_t* accepted,
+                               size_t n_accepted, const uint8_t* alpha, const bte_p2_affine* global_key,
+                               bte_p2_affine* d1_by_party, bte_fp6* total_lines, bte_fp6* mask_lines,
+                               void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t W = total_weight, B = batch_size, T = n_accepted;
+  if (!material || !party_offsets || !accepted || !alpha || !global_key || !d1_by_party || !total_lines ||
+      !mask_lines)
+    return set_error(BTE_EINVAL, "pfe_prepare: null pointer");
+  if (B == 0 || T == 0) return set_error(BTE_EINVAL, "pfe_prepare: empty batch or committee");
+  if (party_offsets[party_count] != W) return set_error(BTE_EINVAL, "pfe_prepare: offsets do not sum to W");
+  std::vector<uint64_t> sel, coff(1, 0);
+  for (size_t p = 0; p < T; ++p) {
+    const uint32_t j = accepted[p];
+    if (j >= party_count) return set_error(BTE_EINVAL, "pfe_prepare: InvalidPartyIndex(%u)", j);
+    for (uint64_t v = party_offsets[j]; v < party_offsets[j + 1]; ++v) sel.push_back(v);
+    coff.push_back(sel.size());
+  }
+  const size_t WT = sel.size();
+  std::vector<bteh::Fr> lam(WT), lam_s(WT);
+  PFE_TRY(bte_lagrange_at_zero(W, sel.data(), WT, (uint8_t*)lam.data()));
+  const bteh::Fr inv2b = bteh::fr_inv(bteh::fr_from_u64(2 * B));
+  for (size_t k = 0; k < WT; ++k) lam_s[k] = bteh::fr_mul(lam[k], inv2b);
+  // d1 MSMs: segment (slot, p) over d1_slot(slot)[range_p] with lambda / 2B
+  std::vector<uint32_t> i1(B * WT), si1(B * WT), i2(B * WT), si2(B * WT);
+  std::vector<uint64_t> o1(B * T + 1), o2(B + 1);
+  for (size_t slot = 0; slot < B; ++slot) {
+    for (size_t p = 0; p < T; ++p) o1[slot * T + p] = slot * WT + coff[p];
+    o2[slot] = slot * WT;
+    for (size_t k = 0; k < WT; ++k) {
+      i1[slot * WT + k] = (uint32_t)(slot * W + sel[k]);
+      si1[slot * WT + k] = (uint32_t)k;
+      i2[slot * WT + k] = (uint32_t)(B * W + slot * W + sel[k]);
+      si2[slot * WT + k] = (uint32_t)k;
+    }
+  }
+  o1[B * T] = B * WT;
+  o2[B] = B * WT;
+  std::vector<bteh::Fr> aa1(B), av(B);
+  for (size_t i = 0; i < B; ++i) {
+    memcpy(av[i].l, alpha + 32 * i, 32);
+    aa1[i] = bteh::fr_mul(av[i], bteh::fr_add(av[i], bteh::fr_from_u64(1)));
+  }
+  if (2 * B * W > 0xffffffffu) return set_error(BTE_EINVAL, "pfe_prepare: material too large");
+  PScratch sc(s);
+  uint8_t *d_lam, *d_lams;
+  uint32_t *d_i1, *d_si1, *d_i2, *d_si2, *d_aa1, *d_a;
+  uint64_t *d_o1, *d_o2;
+  g2j *d1p, *d1t, *d2t, *mask;
+  g2a *d1t_aff, *mask_aff;
+  PFE_TRY(sc.upload(&d_lam, (const uint8_t*)lam.data(), 32 * WT));
+  PFE_TRY(sc.upload(&d_lams, (const uint8_t*)lam_s.data(), 32 * WT));
+  PFE_TRY(sc.upload(&d_i1, i1.data(), i1.size()));
+  PFE_TRY(sc.upload(&d_si1, si1.data(), si1.size()));
+  PFE_TRY(sc.upload(&d_i2, i2.data(), i2.size()));
+  PFE_TRY(sc.upload(&d_si2, si2.data(), si2.size()));
+  PFE_TRY(sc.upload(&d_o1, o1.data(), o1.size()));
+  PFE_TRY(sc.upload(&d_o2, o2.data(), o2.size()));
+  PFE_TRY(sc.upload(&d_aa1, (const uint32_t*)aa1.data(), 8 * B));
+  PFE_TRY(sc.upload(&d_a, (const uint32_t*)av.data(), 8 * B));
+  PFE_TRY(sc.done());
+  PFE_TRY(sc.get(&d1p, B * T));
+  PFE_TRY(sc.get(&d1t, B));
+  PFE_TRY(sc.get(&d2t, B));
+  PFE_TRY(sc.get(&mask, B));
+  PFE_TRY(sc.get(&d1t_aff, B));
+  PFE_TRY(sc.get(&mask_aff, B));
+  const size_t npts = 2 * B * W;
+  PFE_TRY(g2_msm_batched_gather(material, npts, d_i1, d_lams, d_si1, B * WT, d_o1, B * T, 0, d1p, s));
+  PFE_TRY(g2_msm_batched_gather(material, npts, d_i2, d_lam, d_si2, B * WT, d_o2, B, 0, d2t, s));
+  hipLaunchKernelGGL(k_g2_sum_runs, dim3((unsigned)B), dim3(64), 0, s, (const g2j*)d1p, T, B, d1t);
+  hipLaunchKernelGGL(k_pfe_mask, dim3(blocks_for(B, 64)), dim3(64), 0, s, (const g2j*)d2t, (const g2a*)global_key,
+                     (const uint32_t*)d_aa1, (const uint32_t*)d_a, B, mask);
+  PFE_TRY(bte_g2_batch_normalize((const bte_p2*)d1p, B * T, d1_by_party, s));
+  PFE_TRY(bte_g2_batch_normalize((const bte_p2*)d1t, B, (bte_p2_affine*)d1t_aff, s));
+  PFE_TRY(bte_g2_batch_normalize((const bte_p2*)mask, B, (bte_p2_affine*)mask_aff, s));
+  PFE_TRY(bte_g2_prepare_lines((const bte_p2_affine*)d1t_aff, B, total_lines, s));
+  PFE_TRY(bte_g2_prepare_lines((const bte_p2_affine*)mask_aff, B, mask_lines, s));
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+// decryption.rs:540-573 precompute_batch: ct1 = B G1 affine; outputs scaled_first_cauchy (B G1
+// Jacobian) and pairing_cauchy (B Fp12, cyclotomic, before the hard part).
+extern "C" int bte_pfe_precompute(const bte_p1_affine* ct1, size_t batch_size, const bte_fp6* total_lines,
+                                  bte_p1* first_cauchy, bte_fp12* pairing_cauchy, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t B = batch_size;
+  if (!ct1 || !total_lines || !first_cauchy || !pairing_cauchy) return set_error(BTE_EINVAL, "pfe_precompute: null");
+  if (B == 0 || (B & (B - 1))) return set_error(BTE_EINVAL, "pfe_precompute: InvalidBatchSize(%zu)", B);
+  std::vector<int64_t> coef(B);
+  for (size_t k = 0; k < B; ++k) coef[k] = (int64_t)B - 1 - 2 * (int64_t)k;  // decryption.rs:198-200
+  PScratch sc(s);
+  int64_t* d_coef;
+  g1j *c, *spec;
+  fp12 *ml, *cyc, *cspec;
+  PFE_TRY(sc.upload(&d_coef, coef.data(), B));
+  PFE_TRY(sc.done());
+  PFE_TRY(sc.get(&c, B));
+  PFE_TRY(sc.get(&spec, B));
+  PFE_TRY(sc.get(&ml, B));
+  PFE_TRY(sc.get(&cyc, B));
+  PFE_TRY(sc.get(&cspec, B));
+  // cauchy_scaled_group: fft, * (B-1-2k), shift with negation, unscaled ifft
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_jac_from_aff<fp>), dim3(blocks_for(B, 256)), dim3(256), 0, s, (const g1a*)ct1, B, spec);
+  PFE_TRY(bte_fft_g1((bte_p1*)spec, B, 0, 0, s));
+  hipLaunchKernelGGL(k_g1_mul_small, dim3(blocks_for(B, 64)), dim3(64), 0, s, spec, (const int64_t*)d_coef, B);
+  hipLaunchKernelGGL(k_g1_shift_neg, dim3(blocks_for(B, 256)), dim3(256), 0, s, (const g1j*)spec, B, (g1j*)first_cauchy);
+  PFE_TRY(bte_fft_g1(first_cauchy, B, 1, 0, s));
+  // cauchy_scaled_cyclotomic over the easy-exponentiated prepared Miller loops
+  PFE_TRY(bte_miller_loop_prepared(total_lines, ct1, B, (bte_fp12*)ml, s));
+  PFE_TRY(bte_final_exp_easy((const bte_fp12*)ml, B, (bte_fp12*)cyc, s));
+  PFE_TRY(bte_fft_cyclotomic((bte_fp12*)cyc, B, 0, s));
+  PFE_TRY(bte_cyclotomic_pow_small((bte_fp12*)cyc, d_coef, B, s));
+  hipLaunchKernelGGL(k_cyc_shift_conj, dim3(blocks_for(B, 256)), dim3(256), 0, s, (const fp12*)cyc, B, (fp12*)pairing_cauchy);
+  PFE_TRY(bte_fft_cyclotomic(pairing_cauchy, B, 1, s));
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+// decryption.rs:576-644 open_batch.  shares: T G1 Jacobian (committee order); alpha: B canonical
+// scalars (host).  messages_out: B Fp12.
+extern "C" int bte_pfe_open(const bte_p1* shares, size_t n_accepted, const bte_p2_affine* d1_by_party,
+                            const bte_fp6* mask_lines, const bte_p1_affine* ct1, const bte_fp12* ct2,
+                            const bte_p1* first_cauchy, const bte_fp12* pairing_cauchy, const uint8_t* alpha,
+                            size_t batch_size, bte_fp12* messages_out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t B = batch_size, T = n_accepted;
+  if (!shares || !d1_by_party || !mask_lines || !ct1 || !ct2 || !first_cauchy || !pairing_cauchy || !alpha ||
+      !messages_out)
+    return set_error(BTE_EINVAL, "pfe_open: null pointer");
+  if (B == 0 || T == 0) return set_error(BTE_EINVAL, "pfe_open: empty batch or committee");
+  std::vector<uint64_t> off(B + 1);
+  for (size_t i = 0; i <= B; ++i) off[i] = i * T;
+  std::vector<uint8_t> aa1(32 * B);
+  for (size_t i = 0; i < B; ++i) {
+    bteh::Fr a;
+    memcpy(a.l, alpha + 32 * i, 32);
+    bteh::Fr p = bteh::fr_mul(a, bteh::fr_add(a, bteh::fr_from_u64(1)));
+    memcpy(&aa1[32 * i], p.l, 32);
+  }
+  PScratch sc(s);
+  uint64_t* d_off;
+  g1j *sh2, *left;
+  g1a* left_aff;
+  fp12 *ml, *easy, *prod, *comb, *mask;
+  PFE_TRY(sc.upload(&d_off, off.data(), B + 1));
+  PFE_TRY(sc.done());
+  PFE_TRY(sc.get(&sh2, T));
+  PFE_TRY(sc.get(&left, B * T));
+  PFE_TRY(sc.get(&left_aff, B * T));
+  PFE_TRY(sc.get(&ml, 2 * B));
+  PFE_TRY(sc.get(&easy, 2 * B));
+  PFE_TRY(sc.get(&prod, B));
+  PFE_TRY(sc.get(&comb, B));
+  PFE_TRY(sc.get(&mask, B));
+  BTE_HIP(hipMemcpyAsync(sh2, shares, T * sizeof(g1j), hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_g1_mul_small_const, dim3(blocks_for(T, 64)), dim3(64), 0, s, sh2, (int64_t)(2 * B), T);
+  hipLaunchKernelGGL(k_pfe_left, dim3(blocks_for(B * T, 256)), dim3(256), 0, s, (const g1j*)sh2,
+                     (const g1j*)first_cauchy, T, B, left);
+  PFE_TRY(bte_g1_batch_normalize((const bte_p1*)left, B * T, (bte_p1_affine*)left_aff, s));
+  PFE_TRY(bte_multi_miller_loop((const bte_p1_affine*)left_aff, d1_by_party, B * T, d_off, B, (bte_fp12*)ml, s));
+  PFE_TRY(bte_miller_loop_prepared(mask_lines, ct1, B, (bte_fp12*)(ml + B), s));
+  PFE_TRY(bte_final_exp_easy((const bte_fp12*)ml, 2 * B, (bte_fp12*)easy, s));
+  hipLaunchKernelGGL(k_fp12_mul_pairs, dim3(blocks_for(B, 64)), dim3(64), 0, s, (const fp12*)easy,
+                     (const fp12*)pairing_cauchy, B, prod);
+  PFE_TRY(bte_cyclotomic_pow_scalars((bte_fp12*)prod, aa1.data(), B, s));
+  hipLaunchKernelGGL(k_pfe_combine, dim3(blocks_for(B, 64)), dim3(64), 0, s, (const fp12*)(easy + B),
+                     (const fp12*)prod, B, comb);
+  PFE_TRY(bte_final_exp_hard((const bte_fp12*)comb, B, (bte_fp12*)mask, s));
+  hipLaunchKernelGGL(k_pfe_unmask, dim3(blocks_for(B, 64)), dim3(64), 0, s, (const fp12*)ct2, (const fp12*)mask, B,
+                     (fp12*)messages_out);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+// weighted_pfe/src/decryption.rs:230-251 partial_decrypt for N parties at once:
+// sigma_j = sum_i g_{j,i} ct1_i with g_{j,i} given (N x B canonical scalars, device).
+extern "C" int bte_pfe_partial_decrypt(const bte_p1_affine* ct1, size_t batch_size, const uint8_t* fractions,
+                                       size_t n_parties, bte_p1* sigma_out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t B = batch_size, N = n_parties;
+  if (!ct1 || !fractions || !sigma_out) return set_error(BTE_EINVAL, "pfe_partial_decrypt: null pointer");
+  if (N == 0) return BTE_OK;
+  std::vector<uint32_t> pidx(N * B);
+  std::vector<uint64_t> off(N + 1);
+  for (size_t j = 0; j < N; ++j) {
+    off[j] = j * B;
+    for (size_t i = 0; i < B; ++i) pidx[j * B + i] = (uint32_t)i;
+  }
+  off[N] = N * B;
+  PScratch sc(s);
+  uint32_t* d_pidx;
+  uint64_t* d_off;
+  PFE_TRY(sc.upload(&d_pidx, pidx.data(), N * B));
+  PFE_TRY(sc.upload(&d_off, off.data(), N + 1));
+  PFE_TRY(sc.done());
+  PFE_TRY(g1_msm_batched_gather(ct1, B, d_pidx, fractions, nullptr, N * B, d_off, N, 0, sigma_out, s));
+  return BTE_OK;
+}

@@ -102,3 +102,83 @@ class BtxWorkload:
         ct2 = api.cyclotomic_mul_scalar_decomposed(gts, [(a + self.z * b) % R for a, b in zip(s, r)])
         msgs = api.cyclotomic_mul_scalar_decomposed(gts, s)
         return ct1, ct2, msgs
+
+
+class PfeWorkload:
+    """weighted_pfe key material (weighted_pfe/src/setup.rs:149-354) for `weights`, threshold and
+    power-of-two batch size B: device G2 array d1 (B blocks of W) then d2 (B blocks of W), the
+    global key V, alpha points and party fractions g_{j,i} = 1/(rho_j + alpha_i)."""
+
+    def __init__(self, weights, threshold, batch_size, seed=1):
+        st = inputs.Stream(seed, 5000)
+        B = batch_size
+        self.weights = list(weights)
+        self.N = len(weights)
+        self.W = sum(weights)
+        self.t = threshold
+        self.B = B
+        self.offsets = [0]
+        for w in weights:
+            self.offsets.append(self.offsets[-1] + w)
+        owner = [j for j in range(self.N) for _ in range(weights[j])]
+        dom = 1
+        while dom < self.W:
+            dom <<= 1
+        g = pow(ROOT_OF_UNITY, (1 << 32) // dom, R)
+        xs = [pow(g, v, R) for v in range(self.W)]
+        poly = st.scalars(threshold + 1)
+        poly[0] = poly[0] or 1
+        self.z = poly[0]
+        zv = []
+        for x in xs:
+            acc = 0
+            for c in reversed(poly):
+                acc = (acc * x + c) % R
+            zv.append(acc)
+        # alpha_i = omega_B^i + c with the smallest c >= 2 (setup.rs:605-618)
+        gb = pow(ROOT_OF_UNITY, (1 << 32) // B, R)
+        roots = [pow(gb, i, R) for i in range(B)]
+        c = 2
+        while any((r + c) % R == 0 or (r + c + 1) % R == 0 for r in roots):
+            c += 1
+        self.alpha = [(r + c) % R for r in roots]
+        bm1, b0 = st.scalars(2)
+        self.beta_m1, self.beta_0 = bm1 or 1, b0 or 1
+        self.rho = []
+        labels = set()
+        while len(self.rho) < self.N:
+            rho = st.scalars(1)[0]
+            lab = [(rho + a) % R for a in self.alpha]
+            if any(x == 0 or x in labels for x in lab):
+                continue
+            labels.update(lab)
+            self.rho.append(rho)
+        self.g = [[pow((rho + a) % R, -1, R) for a in self.alpha] for rho in self.rho]
+        ainv = [pow(a, -1, R) for a in self.alpha]
+        a1inv = [pow(a + 1, -1, R) for a in self.alpha]
+        u = [[(gj[i] * gj[i] + self.beta_m1 * a1inv[i] + self.beta_0 * ainv[i]) % R for i in range(B)]
+             for gj in self.g]
+        d1 = b"".join((zv[v] * self.g[owner[v]][i] % R).to_bytes(32, "little")
+                      for i in range(B) for v in range(self.W))
+        d2 = b"".join((zv[v] * u[owner[v]][i] % R).to_bytes(32, "little")
+                      for i in range(B) for v in range(self.W))
+        self.material = api.g2_generator_mul(api.to_device(d1 + d2))
+        vkey = self.z * (self.beta_m1 + self.beta_0) % R
+        self.global_key = api.g2_generator_mul(api.to_device(vkey.to_bytes(32, "little")))
+        self.zv = zv
+        self.owner = owner
+        self.stream = st
+        self.gt = gt_generator_bytes()
+
+    def encrypt(self):
+        """B messages gT^s with ct1 = r G1 and ct2 = gT^(s + z beta_0 r) (encryption key
+        gT^(z beta_0))."""
+        B = self.B
+        self.s = self.stream.scalars(B)
+        self.r = self.stream.scalars(B)
+        ct1 = api.g1_generator_mul(api.to_device(inputs.pack_scalars(self.r)))
+        gts = api.to_device(self.gt * B)
+        e = self.z * self.beta_0 % R
+        ct2 = api.cyclotomic_mul_scalar_decomposed(gts, [(a + e * b) % R for a, b in zip(self.s, self.r)])
+        msgs = api.cyclotomic_mul_scalar_decomposed(gts, self.s)
+        return ct1, ct2, msgs
