@@ -175,7 +175,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--logn", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--decrypt-batches", type=int, default=64,
+    ap.add_argument("--decrypt-batches", type=int, default=128,
                     help="batches of 512 ciphertexts in the fused decrypt measurement (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
