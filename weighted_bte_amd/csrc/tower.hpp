@@ -9,9 +9,11 @@
 
 namespace bte {
 
-// Out-of-line device functions: the Fp2 products and every Fp12-level operation are real
-// calls, so kernels built from them (Miller loops, final exponentiation, GT transforms) stay
-// small enough for the instruction cache and for the compiler.
+// Out-of-line device functions: every Fp12-level operation (and the inversions) is a real call
+// whose body has all Fp / Fp2 arithmetic inlined, so a call costs a few hundred bytes of
+// argument traffic against tens of thousands of instructions of work, and kernels built from
+// them (Miller loops, final exponentiation, GT transforms) stay small enough for the
+// instruction cache and for the compiler.
 #define BTE_NOINL static __device__ __attribute__((noinline))
 
 struct fp2 {
@@ -39,7 +41,7 @@ BTE_DEV fp2 fp2_conj(const fp2& a) { return {a.c0, fp_neg(a.c1)}; }
 BTE_DEV fp2 fp2_half(const fp2& a) { return {fp_half(a.c0), fp_half(a.c1)}; }
 
 // Karatsuba: 3 Fp multiplications.
-BTE_NOINL fp2 fp2_mul(const fp2& a, const fp2& b) {
+BTE_DEV fp2 fp2_mul(const fp2& a, const fp2& b) {
   fp t0 = fp_mul(a.c0, b.c0);
   fp t1 = fp_mul(a.c1, b.c1);
   fp t2 = fp_mul(fp_add(a.c0, a.c1), fp_add(b.c0, b.c1));
@@ -47,7 +49,7 @@ BTE_NOINL fp2 fp2_mul(const fp2& a, const fp2& b) {
 }
 
 // (a0 + a1 u)^2 = (a0 + a1)(a0 - a1) + 2 a0 a1 u: 2 Fp multiplications.
-BTE_NOINL fp2 fp2_sqr(const fp2& a) {
+BTE_DEV fp2 fp2_sqr(const fp2& a) {
   fp t = fp_mul(a.c0, a.c1);
   return {fp_mul(fp_add(a.c0, a.c1), fp_sub(a.c0, a.c1)), fp_dbl(t)};
 }
