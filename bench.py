@@ -168,6 +168,56 @@ def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batche
     }
 
 
+def bench_serving(n_requests=48, rate=40.0, max_batch=256, seed=7):
+    """Poisson arrivals of batch-decryption requests (B log-uniform in 16..max_batch) against
+    three resident committees (n log-uniform in 16..1000 scaled to W <= 2048, Pareto(1.16)
+    weights), admitted by HBM budget and fused per (committee, padded batch size)."""
+    import math
+    import random
+
+    import torch
+
+    from weighted_bte_amd import api, serving, workloads
+
+    t0 = time.time()
+    rnd = random.Random(seed)
+    wls, coms = [], []
+    for k in range(3):
+        n = int(round(math.exp(rnd.uniform(math.log(16), math.log(1000)))))
+        W = min(2048, max(n, 4 * n))
+        weights = workloads.pareto_weights(n, W, seed=seed + k)
+        wl = workloads.BtxWorkload(weights, W // 2, max_batch, seed=seed + 10 * k)
+        committee = workloads.heaviest_first(weights, wl.t)
+        wls.append((wl, committee))
+        coms.append(serving.Committee(wl.material, wl.W, wl.L, wl.offsets, committee))
+    reqs, expected = [], []
+    for rid in range(n_requests):
+        cid = rnd.randrange(3)
+        B = int(round(math.exp(rnd.uniform(math.log(16), math.log(max_batch)))))
+        wl, committee = wls[cid]
+        ct1, ct2, msgs = wl.encrypt(B)
+        sig = api.btx_partial_decrypt(ct1, wl.q)
+        shares = torch.cat([sig[144 * j:144 * (j + 1)] for j in committee])
+        reqs.append(serving.Request(rid, cid, B, ct1, ct2, shares))
+        expected.append(msgs)
+    for com in coms:  # committee preparation happens before traffic (resident state)
+        for bp in (16, 32, 64, 128, 256):
+            if bp <= max_batch:
+                com.prepare(bp)
+    torch.cuda.synchronize()
+    t_setup = time.time() - t0
+    srv = serving.BtxServer(coms, hbm_budget=8 << 30, max_fuse=128)
+    p50, p99, rate_ct, wall = serving.run_poisson(srv, reqs, rate)
+    ok = all(torch.equal(r.messages, e) for r, e in zip(reqs, expected))
+    return {"workload": "weighted_btx serving (Poisson arrivals, continuous batching)",
+            "requests": n_requests, "offered_rate_req_per_s": rate, "committees": len(coms),
+            "committee_sizes": [len(c.accepted) for c in coms],
+            "total_weights": [c.total_weight for c in coms], "max_batch": max_batch,
+            "ciphertexts": sum(r.batch for r in reqs), "plaintexts_bit_exact": bool(ok),
+            "p50_latency_ms": round(p50 * 1e3, 1), "p99_latency_ms": round(p99 * 1e3, 1),
+            "ctxts_per_s": round(rate_ct, 1), "wall_s": round(wall, 2), "setup_s": round(t_setup, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +225,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--logn", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=48,
+                    help="requests in the serving measurement (0: skip)")
     ap.add_argument("--decrypt-batches", type=int, default=128,
                     help="batches of 512 ciphertexts in the fused decrypt measurement (0: skip)")
     a = ap.parse_args()
@@ -237,6 +289,7 @@ def main():
 
     cpu = None if a.no_cpu else cpu_baseline(a.logn)
     decrypt = bench_decrypt(a.decrypt_batches, with_cpu=not a.no_cpu) if a.decrypt_batches > 0 else None
+    serving = bench_serving(a.serving_requests) if a.serving_requests > 0 else None
 
     line = {
         "metric": "G1 MSM 2^%d ms (decrypted ctxts/s at B=512 reported separately)" % a.logn,
@@ -271,9 +324,12 @@ def main():
         },
         "cpu_baseline": cpu,
         "decrypt": decrypt,
+        "serving": serving,
     }
     print(json.dumps(line))
-    sys.exit(0 if parity and (decrypt is None or decrypt["plaintexts_bit_exact"]) else 1)
+    ok = parity and (decrypt is None or decrypt["plaintexts_bit_exact"]) and \
+        (serving is None or serving["plaintexts_bit_exact"])
+    sys.exit(0 if ok else 1)
 
 
 if __name__ == "__main__":

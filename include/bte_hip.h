@@ -107,6 +107,13 @@ int bte_fft_g2_batched(bte_p2* values, size_t n, size_t count, int inverse, int 
 int bte_fft_cyclotomic_batched(bte_fp12* values, size_t n, size_t count, int inverse, void* stream);
 int bte_ifft_cyclotomic_prefix_unscaled_batched(bte_fp12* values, size_t n, size_t count,
                                                 size_t output_count, void* stream);
+/* `count` transforms of uneven sizes 2^log_sizes[f] (HOST array, each 0..28) stored back to
+ * back; one launch per stage level serves all of them (the reference's per-transform rayon
+ * loops over Radix2Domain::fft / ifft, batched). */
+int bte_fft_g1_segmented(bte_p1* values, const int* log_sizes, size_t count, int inverse,
+                         int scaled, void* stream);
+int bte_fft_g2_segmented(bte_p2* values, const int* log_sizes, size_t count, int inverse,
+                         int scaled, void* stream);
 /* weighted_pfe/src/fft.rs:324-331 fft_cyclotomic / ifft_cyclotomic_unscaled */
 int bte_fft_cyclotomic(bte_fp12* values, size_t n, int inverse, void* stream);
 /* weighted_btx/src/fft.rs:284-350 ifft_cyclotomic_prefix_unscaled: values[0..output_count)

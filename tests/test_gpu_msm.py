@@ -208,3 +208,15 @@ def test_batched_msm_uneven_segments(api, group):
         for sgm in range(len(sizes)):
             want = sum(ks[idx[t]] * sc[t] for t in range(offs[sgm], offs[sgm + 1]))
             assert read(out[width * sgm: width * (sgm + 1)]) == (mul(want) if want % O.R else None)
+
+
+def test_g2_msm_closed_form_2_16(api):
+    """Config-2 G2 at 2^16 (bases k_i * G2, 10% sparse scalars), compressed-byte equality."""
+    n = 1 << 16
+    rng = O.SeededRng(3, 16)
+    ks = [rng.scalar() for _ in range(n)]
+    sc = O.msm_config_scalars(rng, n)
+    bases = api.g2_generator_mul(api.to_device(pack_scalars(ks)))
+    out = api.g2_multi_exp_affine_bytes(bases, api.to_device(pack_scalars(sc)))
+    want = g2_mul_gen(sum(k * s for k, s in zip(ks, sc)))
+    assert O.g2_compress(g2_jac_affine(api.to_host(out))) == O.g2_compress(want)

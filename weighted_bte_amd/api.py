@@ -218,6 +218,18 @@ class Radix2Domain:
         return v[: output_count * FP12]
 
 
+def fft_g1_segmented(values, log_sizes, inverse=False, scaled=True):
+    """Many G1 transforms of uneven power-of-two sizes (Jacobian, back to back) in one launch
+    per stage level."""
+    v = _dev(values).clone()
+    if v.numel() != sum(1 << l for l in log_sizes) * G1_JAC:
+        raise ValueError("values.len() != sum of sizes")
+    arr = (ctypes.c_int * len(log_sizes))(*log_sizes)
+    call("bte_fft_g1_segmented", _ptr(v), arr, len(log_sizes), 1 if inverse else 0, 1 if scaled else 0,
+         _stream())
+    return v
+
+
 def cyclotomic_mul_scalar_decomposed(values, scalars):
     """fft.rs:65-71, element-wise; scalars are canonical ints (host)."""
     v = _dev(values).clone()

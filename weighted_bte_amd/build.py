@@ -1,8 +1,9 @@
 """Builds weighted_bte_amd/lib/libbte_hip.so for gfx950 with hipcc (in-tree, no JIT cache).
 
-    python -m weighted_bte_amd.build [--jobs N]
+    python -m weighted_bte_amd.build [--jobs N] [--incremental]
 
 Each csrc/*.hip file compiles to an object in csrc/_build/, then one shared library is linked.
+With --incremental an object newer than its source and every header is kept.
 All HIP code is compiled with packed-FP32 instruction selection disabled.
 """
 from __future__ import annotations
@@ -45,13 +46,23 @@ def compile_one(src):
     return obj
 
 
-def build(jobs=None, verbose=True):
+def up_to_date(src):
+    obj = os.path.join(BUILD, src[:-4] + ".o")
+    if not os.path.exists(obj):
+        return False
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+    return all(os.path.getmtime(d) <= os.path.getmtime(obj) for d in deps)
+
+
+def build(jobs=None, verbose=True, incremental=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = sources()
+    todo = [s for s in srcs if not (incremental and up_to_date(s))]
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(compile_one, srcs))
+        list(ex.map(compile_one, todo))
+    objs = [os.path.join(BUILD, s[:-4] + ".o") for s in srcs]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -66,9 +77,10 @@ def build(jobs=None, verbose=True):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--incremental", action="store_true")
     a = ap.parse_args()
     try:
-        build(a.jobs)
+        build(a.jobs, incremental=a.incremental)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

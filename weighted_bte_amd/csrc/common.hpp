@@ -45,6 +45,11 @@ int g2_msm_batched_gather(const void* points, size_t n_points, const uint32_t* p
                           const uint32_t* scalar_index, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
                           int c, void* out, hipStream_t s);
 
+// Segmented cyclotomic prefix inverse transforms (fft.hip).
+struct fp12;
+int cyc_prefix_ifft_segmented(fp12* v, const int* log_sizes, const uint32_t* out_counts, size_t count,
+                              hipStream_t s);
+
 inline unsigned blocks_for(size_t n, unsigned threads) {
   return (unsigned)((n + threads - 1) / threads);
 }

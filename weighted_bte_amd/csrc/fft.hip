@@ -10,6 +10,7 @@
 //
 // Every stage is one launch with one thread per butterfly; twiddles w^j (j < n/2) and their
 // cyclotomic plans are built once per (size, direction) on the host and kept in device memory.
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -287,6 +288,201 @@ static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool sc
   return BTE_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Segmented transforms: `count` transforms of uneven power-of-two sizes stored back to back.
+// Stage lh (butterfly size 2^(lh+1)) uses twiddles w_{2h}^k, which do not depend on the
+// transform size, so one table for the largest size serves all: index k << (logmax-1-lh).
+// ---------------------------------------------------------------------------------------------
+BTE_DEV uint32_t seg_of(const uint64_t* __restrict__ off, uint32_t count, uint64_t g) {
+  uint32_t lo = 0, hi = count;  // off[lo] <= g < off[hi]
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= g) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <class T>
+__global__ void k_bit_reverse_seg(T* v, const uint64_t* __restrict__ off, const uint8_t* __restrict__ logs,
+                                  uint32_t count, uint64_t total) {
+  const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const uint32_t f = seg_of(off, count, g);
+  const int lg = logs[f];
+  T* b = v + off[f];
+  const size_t i = g - off[f];
+  const size_t r = bitrev(i, lg);
+  if (i < r) {
+    T a = b[i];
+    b[i] = b[r];
+    b[r] = a;
+  }
+}
+
+// half_off[f] = off[f] / 2: butterfly slots of transform f.
+template <class F>
+__global__ void __launch_bounds__(64) k_group_stage_seg(jac<F>* __restrict__ v, const uint64_t* __restrict__ off,
+                                                        const uint64_t* __restrict__ half_off,
+                                                        const uint8_t* __restrict__ logs, uint32_t count,
+                                                        uint64_t total_half, int logmax, int log_half,
+                                                        const uint32_t* __restrict__ tw) {
+  const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (g >= total_half) return;
+  const uint32_t f = seg_of(half_off, count, g);
+  if (logs[f] <= log_half) return;  // this transform has no such stage
+  jac<F>* b = v + off[f];
+  const size_t t = g - half_off[f];
+  const size_t half = (size_t)1 << log_half;
+  const size_t k = t & (half - 1);
+  const size_t i = ((t >> log_half) << (log_half + 1)) + k;
+  const jac<F> u = b[i];
+  jac<F> w = b[i + half];
+  if (k) w = group_scalar_mul(w, tw + 8 * (k << (logmax - 1 - log_half)));
+  b[i] = jac_add(u, w);
+  b[i + half] = jac_sub(u, w);
+}
+
+template <class F>
+__global__ void __launch_bounds__(64) k_group_scale_seg(jac<F>* __restrict__ v, const uint64_t* __restrict__ off,
+                                                        uint32_t count, uint64_t total,
+                                                        const uint32_t* __restrict__ inv /* count x 8 */) {
+  const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const uint32_t f = seg_of(off, count, g);
+  v[g] = group_scalar_mul(v[g], inv + 8 * f);
+}
+
+template <class F>
+static int group_fft_segmented(jac<F>* v, const int* log_sizes, size_t count, bool inverse, bool scaled,
+                               hipStream_t s) {
+  if (count == 0) return BTE_OK;
+  if (!v || !log_sizes) return set_error(BTE_EINVAL, "fft_segmented: null pointer");
+  std::vector<uint64_t> off(count + 1), hoff(count + 1);
+  std::vector<uint8_t> logs(count);
+  std::vector<bteh::Fr> inv(count);
+  int logmax = 0;
+  for (size_t f = 0; f < count; ++f) {
+    if (log_sizes[f] < 0 || log_sizes[f] > 28) return set_error(BTE_EINVAL, "fft_segmented: InvalidBatchSize(2^%d)", log_sizes[f]);
+    logs[f] = (uint8_t)log_sizes[f];
+    logmax = std::max(logmax, log_sizes[f]);
+    off[f + 1] = off[f] + ((uint64_t)1 << log_sizes[f]);
+    hoff[f + 1] = off[f + 1] / 2;
+    inv[f] = bteh::fr_inv(bteh::fr_from_u64((uint64_t)1 << log_sizes[f]));
+  }
+  for (size_t f = 0; f <= count; ++f) hoff[f] = off[f] / 2;
+  if (logmax == 0) return BTE_OK;
+  DomainTables* t;
+  int rc = domain_tables(logmax, inverse, false, &t, s);
+  if (rc) return rc;
+  uint64_t *d_off, *d_hoff;
+  uint8_t* d_logs;
+  uint32_t* d_inv;
+  BTE_HIP(hipMallocAsync((void**)&d_off, (count + 1) * 8, s));
+  BTE_HIP(hipMallocAsync((void**)&d_hoff, (count + 1) * 8, s));
+  BTE_HIP(hipMallocAsync((void**)&d_logs, count, s));
+  BTE_HIP(hipMallocAsync((void**)&d_inv, count * 32, s));
+  BTE_HIP(hipMemcpyAsync(d_off, off.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
+  BTE_HIP(hipMemcpyAsync(d_hoff, hoff.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
+  BTE_HIP(hipMemcpyAsync(d_logs, logs.data(), count, hipMemcpyHostToDevice, s));
+  BTE_HIP(hipMemcpyAsync(d_inv, inv.data(), count * 32, hipMemcpyHostToDevice, s));
+  BTE_HIP(hipStreamSynchronize(s));  // host vectors are temporaries
+  const uint64_t total = off[count];
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse_seg<jac<F>>), dim3(blocks_for(total, 256)), dim3(256), 0, s, v,
+                     (const uint64_t*)d_off, (const uint8_t*)d_logs, (uint32_t)count, total);
+  for (int lh = 0; lh < logmax; ++lh)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage_seg<F>), dim3(blocks_for(total / 2, 64)), dim3(64), 0, s, v,
+                       (const uint64_t*)d_off, (const uint64_t*)d_hoff, (const uint8_t*)d_logs, (uint32_t)count,
+                       total / 2, logmax, lh, (const uint32_t*)t->twiddles);
+  if (inverse && scaled)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_scale_seg<F>), dim3(blocks_for(total, 64)), dim3(64), 0, s, v,
+                       (const uint64_t*)d_off, (uint32_t)count, total, (const uint32_t*)d_inv);
+  (void)hipFreeAsync(d_off, s);
+  (void)hipFreeAsync(d_hoff, s);
+  (void)hipFreeAsync(d_logs, s);
+  (void)hipFreeAsync(d_inv, s);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+// Segmented unscaled inverse cyclotomic prefix transforms (fft.rs:284-350) over transforms of
+// sizes 2^logs[f] stored back to back, keeping outs[f] outputs of transform f.  At stage level
+// lh a transform with lh < logs[f]-1 runs a full stage and one with lh == logs[f]-1 runs its
+// truncated last stage; plans come from the inverse domain of the largest size.
+__global__ void __launch_bounds__(64) k_cyc_prefix_stage_seg(fp12* __restrict__ v, const uint64_t* __restrict__ off,
+                                                             const uint64_t* __restrict__ half_off,
+                                                             const uint8_t* __restrict__ logs,
+                                                             const uint32_t* __restrict__ outs, uint32_t count,
+                                                             uint64_t total_half, int logmax, int log_half,
+                                                             const cyc_plan* __restrict__ plans) {
+  const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (g >= total_half) return;
+  const uint32_t f = seg_of(half_off, count, g);
+  const int lg = logs[f];
+  if (lg - 1 < log_half) return;
+  fp12* b = v + off[f];
+  const size_t t = g - half_off[f];
+  const size_t half = (size_t)1 << log_half;
+  const size_t k = t & (half - 1);
+  const cyc_plan* pl = plans + (k << (logmax - 1 - log_half));
+  if (lg - 1 == log_half) {  // truncated last stage: outputs k < outs[f] of the left half only
+    if (k >= outs[f]) return;
+    fp12 w = b[half + k];
+    if (k) w = cyclo_pow_plan(w, pl);
+    b[k] = fp12_mul(b[k], w);
+    return;
+  }
+  const size_t i = ((t >> log_half) << (log_half + 1)) + k;
+  const fp12 u = b[i];
+  fp12 w = b[i + half];
+  if (k) w = cyclo_pow_plan(w, pl);
+  b[i] = fp12_mul(u, w);
+  b[i + half] = fp12_mul(u, fp12_conj(w));
+}
+
+int cyc_prefix_ifft_segmented(fp12* v, const int* log_sizes, const uint32_t* out_counts, size_t count,
+                              hipStream_t s) {
+  if (count == 0) return BTE_OK;
+  std::vector<uint64_t> off(count + 1), hoff(count + 1);
+  std::vector<uint8_t> logs(count);
+  int logmax = 0;
+  for (size_t f = 0; f < count; ++f) {
+    if (log_sizes[f] < 1 || log_sizes[f] > 28) return set_error(BTE_EINVAL, "prefix_segmented: size 2^%d", log_sizes[f]);
+    if (out_counts[f] > (1u << (log_sizes[f] - 1))) return set_error(BTE_EINVAL, "prefix_segmented: output count");
+    logs[f] = (uint8_t)log_sizes[f];
+    logmax = std::max(logmax, log_sizes[f]);
+    off[f + 1] = off[f] + ((uint64_t)1 << log_sizes[f]);
+  }
+  for (size_t f = 0; f <= count; ++f) hoff[f] = off[f] / 2;
+  DomainTables* t;
+  int rc = domain_tables(logmax, true, true, &t, s);
+  if (rc) return rc;
+  uint64_t *d_off, *d_hoff;
+  uint8_t* d_logs;
+  uint32_t* d_outs;
+  BTE_HIP(hipMallocAsync((void**)&d_off, (count + 1) * 8, s));
+  BTE_HIP(hipMallocAsync((void**)&d_hoff, (count + 1) * 8, s));
+  BTE_HIP(hipMallocAsync((void**)&d_logs, count, s));
+  BTE_HIP(hipMallocAsync((void**)&d_outs, count * 4, s));
+  BTE_HIP(hipMemcpyAsync(d_off, off.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
+  BTE_HIP(hipMemcpyAsync(d_hoff, hoff.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
+  BTE_HIP(hipMemcpyAsync(d_logs, logs.data(), count, hipMemcpyHostToDevice, s));
+  BTE_HIP(hipMemcpyAsync(d_outs, out_counts, count * 4, hipMemcpyHostToDevice, s));
+  BTE_HIP(hipStreamSynchronize(s));
+  const uint64_t total = off[count];
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse_seg<fp12>), dim3(blocks_for(total, 256)), dim3(256), 0, s, v,
+                     (const uint64_t*)d_off, (const uint8_t*)d_logs, (uint32_t)count, total);
+  for (int lh = 0; lh < logmax; ++lh)
+    hipLaunchKernelGGL(k_cyc_prefix_stage_seg, dim3(blocks_for(total / 2, 64)), dim3(64), 0, s, v,
+                       (const uint64_t*)d_off, (const uint64_t*)d_hoff, (const uint8_t*)d_logs,
+                       (const uint32_t*)d_outs, (uint32_t)count, total / 2, logmax, lh, (const cyc_plan*)t->plans);
+  (void)hipFreeAsync(d_off, s);
+  (void)hipFreeAsync(d_hoff, s);
+  (void)hipFreeAsync(d_logs, s);
+  (void)hipFreeAsync(d_outs, s);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
 // Plans for caller-given canonical scalars, uploaded to a stream-ordered buffer.
 static int upload_plans(const uint8_t* host_scalars, size_t n, cyc_plan** out, hipStream_t s) {
   std::vector<cyc_plan> pl(n);
@@ -369,4 +565,17 @@ extern "C" int bte_cyclotomic_pow_small(bte_fp12* values, const int64_t* coeffic
   hipLaunchKernelGGL(k_cyc_pow_small, dim3(blocks_for(n, 64)), dim3(64), 0, s, (fp12*)values, n, coefficients);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
+}
+
+// `count` transforms of sizes 2^log_sizes[f] (host array) stored back to back, all stages of
+// all of them in one launch per stage level.
+extern "C" int bte_fft_g1_segmented(bte_p1* values, const int* log_sizes, size_t count, int inverse, int scaled,
+                                    void* stream) {
+  return group_fft_segmented<fp>((jac<fp>*)values, log_sizes, count, inverse != 0, scaled != 0, (hipStream_t)stream);
+}
+
+extern "C" int bte_fft_g2_segmented(bte_p2* values, const int* log_sizes, size_t count, int inverse, int scaled,
+                                    void* stream) {
+  return group_fft_segmented<fp2>((jac<fp2>*)values, log_sizes, count, inverse != 0, scaled != 0,
+                                  (hipStream_t)stream);
 }
