@@ -119,6 +119,12 @@ int bte_fft_cyclotomic(bte_fp12* values, size_t n, int inverse, void* stream);
 /* weighted_btx/src/fft.rs:284-350 ifft_cyclotomic_prefix_unscaled: values[0..output_count)
  * receive the first outputs (output_count <= n/2); the rest of the buffer is scratch. */
 int bte_ifft_cyclotomic_prefix_unscaled(bte_fp12* values, size_t n, size_t output_count, void* stream);
+/* ifft_cyclotomic_prefix_unscaled over `count` transforms of uneven sizes 2^log_sizes[f]
+ * stored back to back (HOST arrays); transform f keeps output_counts[f] <= 2^(log_sizes[f]-1)
+ * outputs at its start. */
+int bte_ifft_cyclotomic_prefix_unscaled_segmented(bte_fp12* values, const int* log_sizes,
+                                                  const uint32_t* output_counts, size_t count,
+                                                  void* stream);
 /* weighted_btx/src/fft.rs:65-71 cyclotomic_mul_scalar_decomposed, element-wise:
  * values[i] <- decomposed action of scalars[i] (canonical 32-byte LE, HOST memory). */
 int bte_cyclotomic_pow_scalars(bte_fp12* values, const uint8_t* host_scalars, size_t n, void* stream);
@@ -139,6 +145,12 @@ int bte_miller_loop_prepared(const bte_fp6* lines, const bte_p1_affine* p, size_
 /* as bte_miller_loop_prepared with pair i using table i % ntables */
 int bte_miller_loop_prepared_tiled(const bte_fp6* lines, size_t ntables, const bte_p1_affine* p,
                                    size_t n, bte_fp12* out, void* stream);
+/* as bte_miller_loop_prepared over several tables: pair i of segment s (seg_offsets[s] <= i <
+ * seg_offsets[s+1]) uses table i - seg_offsets[s] of tables[s].  tables (nseg pointers) and
+ * seg_offsets (nseg+1) are DEVICE arrays. */
+int bte_miller_loop_prepared_segmented(const bte_fp6* const* tables, const uint64_t* seg_offsets,
+                                       size_t nseg, const bte_p1_affine* p, size_t n, bte_fp12* out,
+                                       void* stream);
 /* one unprepared Miller loop per pair (p[i], q[i]) */
 int bte_miller_loop_pairs(const bte_p1_affine* p, const bte_p2_affine* q, size_t n, bte_fp12* out,
                           void* stream);
@@ -173,8 +185,8 @@ int bte_btx_prepare(const bte_p2_affine* material, size_t total_weight, size_t m
                     size_t n_accepted, size_t batch_size, bte_fp6* kernel_lines,
                     bte_p2_affine* opening_keys, void* stream);
 
-/* decryption.rs:555-603 precompute_batch: ct1 = B G1 affine (identity in invalid slots),
- * valid = B HOST flags; beta_out = B Fp12 (device). */
+/* decryption.rs:555-603 precompute_batch: ct1 = B G1 affine, valid = B HOST flags (an invalid
+ * slot's ct1 is read as the identity, as ValidatedBatch stores it); beta_out = B Fp12 (device). */
 int bte_btx_precompute(const bte_p1_affine* ct1, const uint8_t* valid, size_t batch_size,
                        const bte_fp6* kernel_lines, bte_fp12* beta_out, void* stream);
 
@@ -194,6 +206,30 @@ int bte_btx_precompute_batched(const bte_p1_affine* ct1, const uint8_t* valid, s
 int bte_btx_open_batched(const bte_p1* shares, size_t n_accepted, const bte_p2_affine* opening_keys,
                          const bte_fp12* ct2, const bte_fp12* beta, const uint8_t* valid,
                          size_t batch_size, size_t count, bte_fp12* messages_out, void* stream);
+
+/* One decryption request of the serving loop: `count` ciphertexts of a committee whose state was
+ * prepared for batch size `batch_size` >= count (slots count..batch_size-1 are empty, the
+ * reference's None).  Device: ct1 (count G1 affine), ct2 (count Fp12), kernel_lines and
+ * opening_keys from bte_btx_prepare(batch_size), shares (n_accepted G1 Jacobian), messages_out
+ * (count Fp12).  HOST: valid (count flags, NULL = all valid). */
+typedef struct {
+  const bte_p1_affine* ct1;
+  const bte_fp12* ct2;
+  const uint8_t* valid;
+  size_t count;
+  size_t batch_size;
+  const bte_fp6* kernel_lines;
+  const bte_p2_affine* opening_keys;
+  const bte_p1* shares;
+  size_t n_accepted;
+  bte_fp12* messages_out;
+} bte_btx_request;
+
+/* decryption.rs:555-603 precompute_batch followed by :607-695 open_batch for requests of
+ * different committees and batch sizes at once: every stage (segmented G1 FFT, prepared Miller
+ * loops, easy part, segmented prefix IFFT, hard part, uneven multi-Miller loops, final
+ * exponentiation, combine) is one launch over all requests. */
+int bte_btx_decrypt_requests(const bte_btx_request* requests, size_t count, void* stream);
 
 /* decryption.rs:205-233 partial_decrypt for N parties at once: sigma_j = sum_i q_j^i ct1_i,
  * i = 1..B.  q = N canonical scalars (HOST); sigma_out = N G1 Jacobian (device). */

@@ -240,3 +240,14 @@ def test_fft_config_sizes_roundtrip(api, group, logn):
     for k in (0, 1, n // 2 + 3, n - 1):
         v = sum(s[j] * pow(od.generator, j * k, O.R) for j in range(n)) % O.R
         assert frm(ya[width * k:width * (k + 1)]) == mul(v)
+
+
+def test_cyclotomic_prefix_ifft_segmented(api):
+    """Uneven prefix transforms in one launch per stage level equal the per-transform oracle."""
+    rnd = random.Random(70)
+    logs, counts = [1, 3, 2, 3, 4], [1, 2, 0, 4, 5]
+    segs = [[rand_cyclotomic(rnd) for _ in range(1 << l)] for l in logs]
+    x = api.to_device(f12_bytes([v for seg in segs for v in seg]))
+    got = read_f12(api, api.ifft_cyclotomic_prefix_unscaled_segmented(x, logs, counts), sum(counts))
+    want = [v for seg, c in zip(segs, counts) for v in O.ifft_cyclotomic_prefix_unscaled(seg, c)[:c]]
+    assert got == want

@@ -20,6 +20,8 @@ def make_requests(api, workloads, serving, wls, sizes):
 
 
 def test_serving_alone_equals_mixed(dev):
+    import torch
+
     from weighted_bte_amd import api, serving, workloads
 
     wls = []
@@ -29,16 +31,47 @@ def test_serving_alone_equals_mixed(dev):
     coms = [serving.Committee(wl.material, wl.W, wl.L, wl.offsets, c) for wl, c in wls]
     sizes = [(0, 3), (1, 5), (0, 8), (0, 2), (1, 16), (1, 7), (0, 3), (1, 1)]
     reqs, expected = make_requests(api, workloads, serving, wls, sizes)
-    small = max(r.footprint() for r in reqs) * 3  # forces several admission rounds
+    # an invalid slot: identity first component in the validated batch (so in the parties'
+    # shares too) and it opens to one (the reference's None)
+    r = reqs[3]
+    wl, committee = wls[r.committee]
+    raw = bytearray(api.to_host(r.ct1))
+    raw[96:192] = bytes(96)
+    r.ct1 = api.to_device(bytes(raw))
+    sig = api.btx_partial_decrypt(r.ct1, wl.q)
+    r.shares = torch.cat([sig[144 * j:144 * (j + 1)] for j in committee])
+    r.valid = [True, False]
+    small = max(r.footprint() for r in reqs) * 3  # forces several admissions
     srv = serving.BtxServer(coms, hbm_budget=small, max_fuse=4)
     for r in reqs:
         srv.submit(r)
-    rounds = 0
-    while srv.waiting:
-        srv.step()
-        rounds += 1
-    assert rounds >= 2
+    srv.drain()
+    assert srv.launches >= 2
+    from helpers import O
+    one = O.f12_to_bytes(O.F12_ONE)
     for r, want in zip(reqs, expected):
         mixed = api.to_host(r.messages)
+        if r.valid:
+            want = b"".join(want[576 * i:576 * (i + 1)] if v else one for i, v in enumerate(r.valid))
         assert mixed == want, "request %d (mixed)" % r.rid
         assert api.to_host(srv.decrypt_alone(r)) == mixed, "request %d (alone)" % r.rid
+
+
+def test_decrypt_requests_two_in_flight(dev):
+    """Two fused launches on two streams overlap; each result equals the plain path."""
+    from weighted_bte_amd import api, serving, workloads
+
+    weights = [1, 2, 2, 1]
+    wl = workloads.BtxWorkload(weights, threshold=3, max_batch=8, seed=9)
+    com = serving.Committee(wl.material, wl.W, wl.L, wl.offsets, workloads.heaviest_first(weights, wl.t))
+    reqs, expected = make_requests(api, workloads, serving, [(wl, com.accepted)], [(0, 5), (0, 8), (0, 2)])
+    srv = serving.BtxServer([com], max_fuse=1, max_inflight=2)
+    for r in reqs:
+        srv.submit(r)
+    srv.step()
+    srv.step()
+    assert len(srv.inflight) <= 2
+    srv.drain()
+    assert srv.launches == 3
+    for r, want in zip(reqs, expected):
+        assert api.to_host(r.messages) == want

@@ -70,6 +70,9 @@ _SIGNATURES = {
     "bte_pfe_partial_decrypt": [_vp, _sz, _vp, _sz, _vp, _vp],
     "bte_fft_g1_segmented": [_vp, _vp, _sz, _i, _i, _vp],
     "bte_fft_g2_segmented": [_vp, _vp, _sz, _i, _i, _vp],
+    "bte_ifft_cyclotomic_prefix_unscaled_segmented": [_vp, _vp, _vp, _sz, _vp],
+    "bte_miller_loop_prepared_segmented": [_vp, _vp, _sz, _vp, _sz, _vp, _vp],
+    "bte_btx_decrypt_requests": [_vp, _sz, _vp],
     "bte_profile_enable": [_i],
     "bte_profile_read": [ctypes.c_char_p, _sz],
     "bte_probe_mac_rate": [ctypes.POINTER(ctypes.c_double), _vp],

@@ -401,6 +401,52 @@ def btx_open_batched(shares_jac, n_accepted, opening_keys, ct2, beta, valid, bat
     return out
 
 
+class _BtxRequest(ctypes.Structure):
+    """Mirror of bte_btx_request (include/bte_hip.h)."""
+    _fields_ = [("ct1", ctypes.c_void_p), ("ct2", ctypes.c_void_p), ("valid", ctypes.c_void_p),
+                ("count", ctypes.c_size_t), ("batch_size", ctypes.c_size_t),
+                ("kernel_lines", ctypes.c_void_p), ("opening_keys", ctypes.c_void_p),
+                ("shares", ctypes.c_void_p), ("n_accepted", ctypes.c_size_t),
+                ("messages_out", ctypes.c_void_p)]
+
+
+def btx_decrypt_requests(requests):
+    """precompute_batch + open_batch (decryption.rs:555-695) for requests of different committees
+    and batch sizes in one fused pass.  Each request is a dict with ct1 (count G1 affine), ct2
+    (count Fp12), valid (count bools), batch_size (the size its lines/keys were prepared for),
+    lines, keys, shares (T G1 Jacobian) and n_accepted.  Returns one message tensor per request."""
+    arr = (_BtxRequest * len(requests))()
+    keep, outs = [], []
+    for k, r in enumerate(requests):
+        ct2 = _dev(r["ct2"])
+        n = ct2.numel() // FP12
+        out = _empty(n * FP12, ct2)
+        v = (ctypes.c_uint8 * max(n, 1))(*[1 if x else 0 for x in r["valid"]])
+        keep.append(v)
+        outs.append(out)
+        arr[k] = _BtxRequest(_ptr(_dev(r["ct1"])), _ptr(ct2), ctypes.cast(v, ctypes.c_void_p), n,
+                             r["batch_size"], _ptr(_dev(r["lines"])), _ptr(_dev(r["keys"])),
+                             _ptr(_dev(r["shares"])), r["n_accepted"], _ptr(out))
+    call("bte_btx_decrypt_requests", ctypes.cast(arr, ctypes.c_void_p), len(requests), _stream())
+    return outs
+
+
+def ifft_cyclotomic_prefix_unscaled_segmented(values, log_sizes, output_counts):
+    """Prefix inverse cyclotomic transforms of uneven sizes (back to back); returns the
+    concatenated prefixes."""
+    v = _dev(values).clone()
+    if v.numel() != sum(1 << l for l in log_sizes) * FP12:
+        raise ValueError("values.len() != sum of sizes")
+    la = (ctypes.c_int * len(log_sizes))(*log_sizes)
+    oa = (ctypes.c_uint32 * len(log_sizes))(*output_counts)
+    call("bte_ifft_cyclotomic_prefix_unscaled_segmented", _ptr(v), la, oa, len(log_sizes), _stream())
+    parts, off = [], 0
+    for l, c in zip(log_sizes, output_counts):
+        parts.append(v[off * FP12:(off + c) * FP12])
+        off += 1 << l
+    return torch.cat(parts) if parts else v[:0]
+
+
 def btx_partial_decrypt(ct1_affine, q_scalars):
     """partial_decrypt (decryption.rs:205-233) for every party key q_j -> sigma (N x G1 Jacobian)."""
     ct1 = _dev(ct1_affine)

@@ -137,12 +137,14 @@ __global__ void __launch_bounds__(64) k_scale_all_g2(g2j* __restrict__ v, size_t
   if (i < n) v[i] = group_scalar_mul(v[i], s);
 }
 
-// count batches: out[b m + i] = in[b B + i] (i < B) else the identity.
-__global__ void k_g1_pad(const g1a* __restrict__ in, size_t B, size_t m, size_t count, g1j* __restrict__ out) {
+// count batches: out[b m + i] = in[b B + i] (i < B, slot valid) else the identity (an invalid
+// slot contributes the identity everywhere, decryption.rs:113-125).
+__global__ void k_g1_pad(const g1a* __restrict__ in, const uint8_t* __restrict__ valid, size_t B, size_t m,
+                         size_t count, g1j* __restrict__ out) {
   const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (g >= m * count) return;
   const size_t b = g / m, i = g % m;
-  out[g] = i < B ? jac_from_aff(in[b * B + i]) : jac_identity<fp>();
+  out[g] = (i < B && valid[b * B + i]) ? jac_from_aff(in[b * B + i]) : jac_identity<fp>();
 }
 
 // out[b B + k] = in[b m + k] for k < B.
@@ -184,6 +186,71 @@ __global__ void __launch_bounds__(64) k_open_combine(const fp12* __restrict__ ct
     return;
   }
   out[i] = fp12_mul(fp12_mul(ct2[i], fp12_conj(alpha[i])), beta[i]);
+}
+
+// Device view of one request of bte_btx_decrypt_requests.
+struct ReqDev {
+  const g1a* ct1;
+  const fp12* ct2;
+  const g2a* keys;  // batch_size x T, slot-major
+  const g1j* shares;
+  fp12* out;
+  uint64_t count, T;
+};
+
+// Request k owns m-slots [m_off[k], m_off[k+1]) of the padded FFT input: its ciphertexts' first
+// components (identity for invalid ones), then identities up to nextpow2(2 batch_size).
+__global__ void k_req_pad(const ReqDev* __restrict__ R, const uint64_t* __restrict__ m_off,
+                          const uint64_t* __restrict__ slot_off, const uint8_t* __restrict__ valid, uint32_t K,
+                          uint64_t total, g1j* __restrict__ c) {
+  const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const uint32_t k = seg_of(m_off, K, g);
+  const uint64_t i = g - m_off[k];
+  c[g] = (i < R[k].count && valid[slot_off[k] + i]) ? jac_from_aff(R[k].ct1[i]) : jac_identity<fp>();
+}
+
+// pref[slot_off[k] + i] = conv[m_off[k] + i] for i < count_k.
+__global__ void k_req_gather_prefix(const fp12* __restrict__ conv, const uint64_t* __restrict__ m_off,
+                                    const uint64_t* __restrict__ slot_off, uint32_t K, uint64_t S,
+                                    fp12* __restrict__ pref) {
+  const uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (j >= S) return;
+  const uint32_t k = seg_of(slot_off, K, j);
+  pref[j] = conv[m_off[k] + (j - slot_off[k])];
+}
+
+__global__ void k_req_gather_shares(const ReqDev* __restrict__ R, const uint64_t* __restrict__ share_off,
+                                    uint32_t K, uint64_t total, g1j* __restrict__ out) {
+  const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const uint32_t k = seg_of(share_off, K, g);
+  out[g] = R[k].shares[g - share_off[k]];
+}
+
+// Pair g belongs to slot j (pair_off[j] <= g < pair_off[j+1]; invalid slots own no pairs) of
+// request k; party p = g - pair_off[j]: P = share p of k, Q = opening key (slot, p) of k.
+__global__ void k_req_pairs(const ReqDev* __restrict__ R, const uint64_t* __restrict__ slot_off,
+                            const uint64_t* __restrict__ pair_off, const uint64_t* __restrict__ share_off,
+                            uint32_t K, uint32_t S, uint64_t npairs, const g1a* __restrict__ sa,
+                            g1a* __restrict__ P, g2a* __restrict__ Q) {
+  const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (g >= npairs) return;
+  const uint32_t j = seg_of(pair_off, S, g);
+  const uint32_t k = seg_of(slot_off, K, j);
+  const uint64_t p = g - pair_off[j], i = j - slot_off[k];
+  P[g] = sa[share_off[k] + p];
+  Q[g] = R[k].keys[i * R[k].T + p];
+}
+
+__global__ void __launch_bounds__(64) k_req_combine(const ReqDev* __restrict__ R, const uint64_t* __restrict__ slot_off,
+                                                    uint32_t K, uint64_t S, const uint8_t* __restrict__ valid,
+                                                    const fp12* __restrict__ alpha, const fp12* __restrict__ beta) {
+  const uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (j >= S) return;
+  const uint32_t k = seg_of(slot_off, K, j);
+  const uint64_t i = j - slot_off[k];
+  R[k].out[i] = valid[j] ? fp12_mul(fp12_mul(R[k].ct2[i], fp12_conj(alpha[j])), beta[j]) : fp12_one();
 }
 
 // Stream-ordered scratch that is released when the driver returns.
@@ -371,7 +438,8 @@ extern "C" int bte_btx_precompute_batched(const bte_p1_affine* ct1, const uint8_
   BTE_TRY(sc.uploads_done());
   PhaseTimer pt(s);
   pt.begin();
-  hipLaunchKernelGGL(k_g1_pad, dim3(blocks_for(m * K, 256)), dim3(256), 0, s, (const g1a*)ct1, B, m, K, c);
+  hipLaunchKernelGGL(k_g1_pad, dim3(blocks_for(m * K, 256)), dim3(256), 0, s, (const g1a*)ct1, (const uint8_t*)d_valid, B, m, K,
+                     c);
   BTE_TRY(bte_fft_g1_batched((bte_p1*)c, m, K, 0, 0, s));
   BTE_TRY(bte_g1_batch_normalize((const bte_p1*)c, m * K, (bte_p1_affine*)ca, s));
   pt.mark("g1_fft");
@@ -455,6 +523,113 @@ extern "C" int bte_btx_open(const bte_p1* shares, size_t n_accepted, const bte_p
                             bte_fp12* messages_out, void* stream) {
   return bte_btx_open_batched(shares, n_accepted, opening_keys, ct2, beta, valid, batch_size, 1, messages_out,
                               stream);
+}
+
+extern "C" int bte_btx_decrypt_requests(const bte_btx_request* requests, size_t count, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (count == 0) return BTE_OK;
+  if (!requests) return set_error(BTE_EINVAL, "btx_decrypt_requests: null pointer");
+  std::vector<ReqDev> R;
+  std::vector<const void*> tables;
+  std::vector<int> logs;
+  std::vector<uint32_t> outs;
+  std::vector<uint64_t> m_off{0}, slot_off{0}, share_off{0}, pair_off{0};
+  std::vector<uint8_t> valid;
+  for (size_t k = 0; k < count; ++k) {
+    const bte_btx_request& q = requests[k];
+    if (q.count == 0) continue;
+    if (!q.ct1 || !q.ct2 || !q.kernel_lines || !q.opening_keys || !q.shares || !q.messages_out)
+      return set_error(BTE_EINVAL, "btx_decrypt_requests: null pointer in request %zu", k);
+    if (q.count > q.batch_size || q.n_accepted == 0)
+      return set_error(BTE_EINVAL, "btx_decrypt_requests: request %zu has count > batch_size or no shares", k);
+    const size_t m = next_pow2(2 * q.batch_size);
+    int lg = 0;
+    while (((size_t)1 << lg) < m) ++lg;
+    R.push_back({(const g1a*)q.ct1, (const fp12*)q.ct2, (const g2a*)q.opening_keys, (const g1j*)q.shares,
+                 (fp12*)q.messages_out, q.count, q.n_accepted});
+    tables.push_back(q.kernel_lines);
+    logs.push_back(lg);
+    outs.push_back((uint32_t)q.count);
+    m_off.push_back(m_off.back() + m);
+    slot_off.push_back(slot_off.back() + q.count);
+    share_off.push_back(share_off.back() + q.n_accepted);
+    for (size_t i = 0; i < q.count; ++i) {
+      const uint8_t v = q.valid ? (q.valid[i] != 0) : 1;
+      valid.push_back(v);
+      pair_off.push_back(pair_off.back() + (v ? q.n_accepted : 0));
+    }
+  }
+  const size_t K = R.size();
+  if (K == 0) return BTE_OK;
+  const uint64_t M = m_off.back(), S = slot_off.back(), SH = share_off.back(), NP = pair_off.back();
+  if (S > 0xffffffffull) return set_error(BTE_EINVAL, "btx_decrypt_requests: too many ciphertexts");
+  Scratch sc(s);
+  ReqDev* d_R;
+  const void** d_tables;
+  uint64_t *d_moff, *d_soff, *d_shoff, *d_poff;
+  uint8_t* d_valid;
+  g1j *c, *shj;
+  g1a *ca, *sa, *P;
+  g2a* Q;
+  fp12 *ml, *conv, *pref, *beta, *mm, *alpha;
+  BTE_TRY(sc.upload(&d_R, R.data(), K));
+  BTE_TRY(sc.upload(&d_tables, tables.data(), K));
+  BTE_TRY(sc.upload(&d_moff, m_off.data(), K + 1));
+  BTE_TRY(sc.upload(&d_soff, slot_off.data(), K + 1));
+  BTE_TRY(sc.upload(&d_shoff, share_off.data(), K + 1));
+  BTE_TRY(sc.upload(&d_poff, pair_off.data(), S + 1));
+  BTE_TRY(sc.upload(&d_valid, valid.data(), S));
+  BTE_TRY(sc.get(&c, M));
+  BTE_TRY(sc.get(&ca, M));
+  BTE_TRY(sc.get(&ml, M));
+  BTE_TRY(sc.get(&conv, M));
+  BTE_TRY(sc.get(&pref, S));
+  BTE_TRY(sc.get(&beta, S));
+  BTE_TRY(sc.get(&shj, SH));
+  BTE_TRY(sc.get(&sa, SH));
+  BTE_TRY(sc.get(&P, NP));
+  BTE_TRY(sc.get(&Q, NP));
+  BTE_TRY(sc.get(&mm, S));
+  BTE_TRY(sc.get(&alpha, S));
+  BTE_TRY(sc.uploads_done());
+  PhaseTimer pt(s);
+  pt.begin();
+  hipLaunchKernelGGL(k_req_pad, dim3(blocks_for(M, 256)), dim3(256), 0, s, (const ReqDev*)d_R,
+                     (const uint64_t*)d_moff, (const uint64_t*)d_soff, (const uint8_t*)d_valid, (uint32_t)K, M, c);
+  BTE_TRY(bte_fft_g1_segmented((bte_p1*)c, logs.data(), K, 0, 0, s));
+  BTE_TRY(bte_g1_batch_normalize((const bte_p1*)c, M, (bte_p1_affine*)ca, s));
+  pt.mark("g1_fft");
+  BTE_TRY(miller_prepared_segmented((const void* const*)d_tables, d_moff, K, ca, M, ml, s));
+  pt.mark("miller");
+  BTE_TRY(bte_final_exp_easy((const bte_fp12*)ml, M, (bte_fp12*)conv, s));
+  pt.mark("easy");
+  BTE_TRY(cyc_prefix_ifft_segmented(conv, logs.data(), outs.data(), K, s));
+  hipLaunchKernelGGL(k_req_gather_prefix, dim3(blocks_for(S, 256)), dim3(256), 0, s, (const fp12*)conv,
+                     (const uint64_t*)d_moff, (const uint64_t*)d_soff, (uint32_t)K, S, pref);
+  pt.mark("cyclo_ifft");
+  BTE_TRY(bte_final_exp_hard((const bte_fp12*)pref, S, (bte_fp12*)beta, s));
+  pt.mark("hard");
+  hipLaunchKernelGGL(k_req_gather_shares, dim3(blocks_for(SH, 256)), dim3(256), 0, s, (const ReqDev*)d_R,
+                     (const uint64_t*)d_shoff, (uint32_t)K, SH, shj);
+  BTE_TRY(bte_g1_batch_normalize((const bte_p1*)shj, SH, (bte_p1_affine*)sa, s));
+  if (NP)
+    hipLaunchKernelGGL(k_req_pairs, dim3(blocks_for(NP, 256)), dim3(256), 0, s, (const ReqDev*)d_R,
+                       (const uint64_t*)d_soff, (const uint64_t*)d_poff, (const uint64_t*)d_shoff, (uint32_t)K,
+                       (uint32_t)S, NP, (const g1a*)sa, P, Q);
+  BTE_TRY(bte_multi_miller_loop((const bte_p1_affine*)P, (const bte_p2_affine*)Q, NP, d_poff, S, (bte_fp12*)mm, s));
+  pt.mark("multi_miller");
+  BTE_TRY(bte_final_exp((const bte_fp12*)mm, S, (bte_fp12*)alpha, s));
+  pt.mark("final_exp");
+  hipLaunchKernelGGL(k_req_combine, dim3(blocks_for(S, 64)), dim3(64), 0, s, (const ReqDev*)d_R,
+                     (const uint64_t*)d_soff, (uint32_t)K, S, (const uint8_t*)d_valid, (const fp12*)alpha,
+                     (const fp12*)beta);
+  pt.mark("combine");
+  BTE_CHECK_LAUNCH();
+  if (pt.on) {
+    BTE_HIP(hipStreamSynchronize(s));
+    pt.publish();
+  }
+  return BTE_OK;
 }
 
 // decryption.rs:205-233 partial_decrypt for every party at once: sigma_j = sum_i q_j^i C_i

@@ -50,6 +50,21 @@ struct fp12;
 int cyc_prefix_ifft_segmented(fp12* v, const int* log_sizes, const uint32_t* out_counts, size_t count,
                               hipStream_t s);
 
+// Miller loops where element i of segment s (seg_off[s] <= i < seg_off[s+1]) uses line table
+// i - seg_off[s] of tables[s]; tables and seg_off are device arrays (pairing.hip).
+int miller_prepared_segmented(const void* const* tables, const uint64_t* seg_off, size_t nseg, const void* p,
+                              size_t n, void* out, hipStream_t s);
+
+// Segment holding flat index g: off[f] <= g < off[f + 1], off[0] = 0, count segments.
+__device__ inline uint32_t seg_of(const uint64_t* __restrict__ off, uint32_t count, uint64_t g) {
+  uint32_t lo = 0, hi = count;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= g) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 inline unsigned blocks_for(size_t n, unsigned threads) {
   return (unsigned)((n + threads - 1) / threads);
 }

@@ -10,6 +10,10 @@
 //
 // Every stage is one launch with one thread per butterfly; twiddles w^j (j < n/2) and their
 // cyclotomic plans are built once per (size, direction) on the host and kept in device memory.
+// Butterflies are numbered twiddle-major: the lanes of a wavefront take butterflies that share
+// one twiddle (same exponent, so the same branch pattern of the double-and-add or NAF chain)
+// whenever a stage has at least 64 butterflies per twiddle, instead of 64 different exponents
+// whose union of add steps every lane would wait through.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -155,17 +159,28 @@ template <class T> __global__ void k_bit_reverse(T* v, size_t n, int logn, size_
   }
 }
 
+// Butterfly g of one stage (butterfly size 2 * 2^log_half) over `count` transforms of size n,
+// twiddle-major: the nb * count butterflies (nb blocks per transform) of twiddle k are
+// consecutive.  Returns the index of the butterfly's first element; k receives the twiddle.
+BTE_DEV size_t butterfly_slot(size_t g, size_t n, int log_half, size_t count, size_t& k) {
+  const size_t half = (size_t)1 << log_half;
+  const size_t nb = n >> (log_half + 1);
+  const size_t S = nb * count;
+  k = g / S;
+  const size_t r = g - k * S;
+  const size_t b = r / nb, j = r - b * nb;
+  return b * n + j * 2 * half + k;
+}
+
 // One DIT stage for butterfly size 2 * 2^log_half; twiddle index k << (logn - 1 - log_half).
 template <class F>
 __global__ void __launch_bounds__(64) k_group_stage(jac<F>* __restrict__ v, size_t n, int logn, int log_half,
                                                     const uint32_t* __restrict__ tw, size_t count) {
   const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (g >= (n / 2) * count) return;
-  v += (g >> (logn - 1)) * n;
-  const size_t t = g & (n / 2 - 1);
   const size_t half = (size_t)1 << log_half;
-  const size_t k = t & (half - 1);
-  const size_t i = ((t >> log_half) << (log_half + 1)) + k;
+  size_t k;
+  const size_t i = butterfly_slot(g, n, log_half, count, k);
   const jac<F> u = v[i];
   jac<F> w = v[i + half];
   if (k) w = group_scalar_mul(w, tw + 8 * (k << (logn - 1 - log_half)));
@@ -184,11 +199,9 @@ __global__ void __launch_bounds__(64) k_cyc_stage(fp12* __restrict__ v, size_t n
                                                   const cyc_plan* __restrict__ plans, size_t count) {
   const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (g >= (n / 2) * count) return;
-  v += (g >> (logn - 1)) * n;
-  const size_t t = g & (n / 2 - 1);
   const size_t half = (size_t)1 << log_half;
-  const size_t k = t & (half - 1);
-  const size_t i = ((t >> log_half) << (log_half + 1)) + k;
+  size_t k;
+  const size_t i = butterfly_slot(g, n, log_half, count, k);
   const fp12 u = v[i];
   fp12 w = v[i + half];
   if (k) w = cyclo_pow_plan(w, plans + (k << (logn - 1 - log_half)));
@@ -201,8 +214,8 @@ __global__ void __launch_bounds__(64) k_cyc_prefix_last(fp12* __restrict__ v, si
                                                         const cyc_plan* __restrict__ plans, size_t ntrans) {
   const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (g >= count * ntrans) return;
-  v += (g / count) * n;
-  const size_t k = g % count;
+  const size_t k = g / ntrans;  // twiddle-major
+  v += (g - k * ntrans) * n;
   const size_t half = n / 2;
   fp12 w = v[half + k];
   if (k) w = cyclo_pow_plan(w, plans + k);
@@ -293,15 +306,6 @@ static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool sc
 // Stage lh (butterfly size 2^(lh+1)) uses twiddles w_{2h}^k, which do not depend on the
 // transform size, so one table for the largest size serves all: index k << (logmax-1-lh).
 // ---------------------------------------------------------------------------------------------
-BTE_DEV uint32_t seg_of(const uint64_t* __restrict__ off, uint32_t count, uint64_t g) {
-  uint32_t lo = 0, hi = count;  // off[lo] <= g < off[hi]
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (off[mid] <= g) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
 template <class T>
 __global__ void k_bit_reverse_seg(T* v, const uint64_t* __restrict__ off, const uint8_t* __restrict__ logs,
                                   uint32_t count, uint64_t total) {
@@ -319,27 +323,42 @@ __global__ void k_bit_reverse_seg(T* v, const uint64_t* __restrict__ off, const 
   }
 }
 
-// half_off[f] = off[f] / 2: butterfly slots of transform f.
+// Segmented stages take a host-built list of butterflies (index of the first element) in
+// twiddle-major order: for each twiddle k, every block of every transform that has the stage.
+// With outs != nullptr (prefix inverse transforms), a transform whose last stage this is
+// contributes only its butterflies k < outs[f] (the truncated stage).
+static void stage_order(const std::vector<uint64_t>& off, const std::vector<uint8_t>& logs, const uint32_t* outs,
+                        int lh, std::vector<uint32_t>& order) {
+  const size_t half = (size_t)1 << lh;
+  for (size_t k = 0; k < half; ++k)
+    for (size_t f = 0; f < logs.size(); ++f) {
+      const int lg = logs[f];
+      if (lg - 1 < lh) continue;
+      if (outs && lg - 1 == lh) {
+        if (k < outs[f]) order.push_back((uint32_t)(off[f] + k));
+        continue;
+      }
+      const size_t nb = (size_t)1 << (lg - 1 - lh);
+      for (size_t j = 0; j < nb; ++j) order.push_back((uint32_t)(off[f] + j * 2 * half + k));
+    }
+}
+
 template <class F>
-__global__ void __launch_bounds__(64) k_group_stage_seg(jac<F>* __restrict__ v, const uint64_t* __restrict__ off,
-                                                        const uint64_t* __restrict__ half_off,
-                                                        const uint8_t* __restrict__ logs, uint32_t count,
-                                                        uint64_t total_half, int logmax, int log_half,
+__global__ void __launch_bounds__(64) k_group_stage_ord(jac<F>* __restrict__ v, const uint32_t* __restrict__ order,
+                                                        uint64_t nbf, const uint64_t* __restrict__ off,
+                                                        uint32_t count, int logmax, int log_half,
                                                         const uint32_t* __restrict__ tw) {
   const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (g >= total_half) return;
-  const uint32_t f = seg_of(half_off, count, g);
-  if (logs[f] <= log_half) return;  // this transform has no such stage
-  jac<F>* b = v + off[f];
-  const size_t t = g - half_off[f];
+  if (g >= nbf) return;
+  const size_t i = order[g];
+  const uint32_t f = seg_of(off, count, i);
   const size_t half = (size_t)1 << log_half;
-  const size_t k = t & (half - 1);
-  const size_t i = ((t >> log_half) << (log_half + 1)) + k;
-  const jac<F> u = b[i];
-  jac<F> w = b[i + half];
+  const size_t k = (i - off[f]) & (half - 1);
+  const jac<F> u = v[i];
+  jac<F> w = v[i + half];
   if (k) w = group_scalar_mul(w, tw + 8 * (k << (logmax - 1 - log_half)));
-  b[i] = jac_add(u, w);
-  b[i + half] = jac_sub(u, w);
+  v[i] = jac_add(u, w);
+  v[i + half] = jac_sub(u, w);
 }
 
 template <class F>
@@ -357,7 +376,7 @@ static int group_fft_segmented(jac<F>* v, const int* log_sizes, size_t count, bo
                                hipStream_t s) {
   if (count == 0) return BTE_OK;
   if (!v || !log_sizes) return set_error(BTE_EINVAL, "fft_segmented: null pointer");
-  std::vector<uint64_t> off(count + 1), hoff(count + 1);
+  std::vector<uint64_t> off(count + 1);
   std::vector<uint8_t> logs(count);
   std::vector<bteh::Fr> inv(count);
   int logmax = 0;
@@ -366,40 +385,49 @@ static int group_fft_segmented(jac<F>* v, const int* log_sizes, size_t count, bo
     logs[f] = (uint8_t)log_sizes[f];
     logmax = std::max(logmax, log_sizes[f]);
     off[f + 1] = off[f] + ((uint64_t)1 << log_sizes[f]);
-    hoff[f + 1] = off[f + 1] / 2;
     inv[f] = bteh::fr_inv(bteh::fr_from_u64((uint64_t)1 << log_sizes[f]));
   }
-  for (size_t f = 0; f <= count; ++f) hoff[f] = off[f] / 2;
   if (logmax == 0) return BTE_OK;
+  const uint64_t total = off[count];
+  if (total > 0xffffffffull) return set_error(BTE_EINVAL, "fft_segmented: more than 2^32 elements");
+  std::vector<uint32_t> order;
+  std::vector<size_t> start(logmax + 1, 0);
+  for (int lh = 0; lh < logmax; ++lh) {
+    start[lh] = order.size();
+    stage_order(off, logs, nullptr, lh, order);
+  }
+  start[logmax] = order.size();
   DomainTables* t;
   int rc = domain_tables(logmax, inverse, false, &t, s);
   if (rc) return rc;
-  uint64_t *d_off, *d_hoff;
+  uint64_t* d_off;
   uint8_t* d_logs;
-  uint32_t* d_inv;
+  uint32_t *d_inv, *d_order;
   BTE_HIP(hipMallocAsync((void**)&d_off, (count + 1) * 8, s));
-  BTE_HIP(hipMallocAsync((void**)&d_hoff, (count + 1) * 8, s));
   BTE_HIP(hipMallocAsync((void**)&d_logs, count, s));
   BTE_HIP(hipMallocAsync((void**)&d_inv, count * 32, s));
+  BTE_HIP(hipMallocAsync((void**)&d_order, std::max<size_t>(order.size(), 1) * 4, s));
   BTE_HIP(hipMemcpyAsync(d_off, off.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
-  BTE_HIP(hipMemcpyAsync(d_hoff, hoff.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
   BTE_HIP(hipMemcpyAsync(d_logs, logs.data(), count, hipMemcpyHostToDevice, s));
   BTE_HIP(hipMemcpyAsync(d_inv, inv.data(), count * 32, hipMemcpyHostToDevice, s));
+  if (!order.empty()) BTE_HIP(hipMemcpyAsync(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
   BTE_HIP(hipStreamSynchronize(s));  // host vectors are temporaries
-  const uint64_t total = off[count];
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse_seg<jac<F>>), dim3(blocks_for(total, 256)), dim3(256), 0, s, v,
                      (const uint64_t*)d_off, (const uint8_t*)d_logs, (uint32_t)count, total);
-  for (int lh = 0; lh < logmax; ++lh)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage_seg<F>), dim3(blocks_for(total / 2, 64)), dim3(64), 0, s, v,
-                       (const uint64_t*)d_off, (const uint64_t*)d_hoff, (const uint8_t*)d_logs, (uint32_t)count,
-                       total / 2, logmax, lh, (const uint32_t*)t->twiddles);
+  for (int lh = 0; lh < logmax; ++lh) {
+    const uint64_t nbf = start[lh + 1] - start[lh];
+    if (nbf)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage_ord<F>), dim3(blocks_for(nbf, 64)), dim3(64), 0, s, v,
+                         (const uint32_t*)d_order + start[lh], nbf, (const uint64_t*)d_off, (uint32_t)count, logmax,
+                         lh, (const uint32_t*)t->twiddles);
+  }
   if (inverse && scaled)
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_scale_seg<F>), dim3(blocks_for(total, 64)), dim3(64), 0, s, v,
                        (const uint64_t*)d_off, (uint32_t)count, total, (const uint32_t*)d_inv);
   (void)hipFreeAsync(d_off, s);
-  (void)hipFreeAsync(d_hoff, s);
   (void)hipFreeAsync(d_logs, s);
   (void)hipFreeAsync(d_inv, s);
+  (void)hipFreeAsync(d_order, s);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
@@ -408,41 +436,32 @@ static int group_fft_segmented(jac<F>* v, const int* log_sizes, size_t count, bo
 // sizes 2^logs[f] stored back to back, keeping outs[f] outputs of transform f.  At stage level
 // lh a transform with lh < logs[f]-1 runs a full stage and one with lh == logs[f]-1 runs its
 // truncated last stage; plans come from the inverse domain of the largest size.
-__global__ void __launch_bounds__(64) k_cyc_prefix_stage_seg(fp12* __restrict__ v, const uint64_t* __restrict__ off,
-                                                             const uint64_t* __restrict__ half_off,
-                                                             const uint8_t* __restrict__ logs,
-                                                             const uint32_t* __restrict__ outs, uint32_t count,
-                                                             uint64_t total_half, int logmax, int log_half,
-                                                             const cyc_plan* __restrict__ plans) {
+__global__ void __launch_bounds__(64) k_cyc_stage_ord(fp12* __restrict__ v, const uint32_t* __restrict__ order,
+                                                      uint64_t nbf, const uint64_t* __restrict__ off,
+                                                      const uint8_t* __restrict__ logs, uint32_t count, int logmax,
+                                                      int log_half, const cyc_plan* __restrict__ plans) {
   const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (g >= total_half) return;
-  const uint32_t f = seg_of(half_off, count, g);
-  const int lg = logs[f];
-  if (lg - 1 < log_half) return;
-  fp12* b = v + off[f];
-  const size_t t = g - half_off[f];
+  if (g >= nbf) return;
+  const size_t i = order[g];
+  const uint32_t f = seg_of(off, count, i);
   const size_t half = (size_t)1 << log_half;
-  const size_t k = t & (half - 1);
+  const size_t k = (i - off[f]) & (half - 1);
   const cyc_plan* pl = plans + (k << (logmax - 1 - log_half));
-  if (lg - 1 == log_half) {  // truncated last stage: outputs k < outs[f] of the left half only
-    if (k >= outs[f]) return;
-    fp12 w = b[half + k];
-    if (k) w = cyclo_pow_plan(w, pl);
-    b[k] = fp12_mul(b[k], w);
+  fp12 w = v[i + half];
+  if (k) w = cyclo_pow_plan(w, pl);
+  if (logs[f] - 1 == log_half) {  // truncated last stage: left output only
+    v[i] = fp12_mul(v[i], w);
     return;
   }
-  const size_t i = ((t >> log_half) << (log_half + 1)) + k;
-  const fp12 u = b[i];
-  fp12 w = b[i + half];
-  if (k) w = cyclo_pow_plan(w, pl);
-  b[i] = fp12_mul(u, w);
-  b[i + half] = fp12_mul(u, fp12_conj(w));
+  const fp12 u = v[i];
+  v[i] = fp12_mul(u, w);
+  v[i + half] = fp12_mul(u, fp12_conj(w));
 }
 
 int cyc_prefix_ifft_segmented(fp12* v, const int* log_sizes, const uint32_t* out_counts, size_t count,
                               hipStream_t s) {
   if (count == 0) return BTE_OK;
-  std::vector<uint64_t> off(count + 1), hoff(count + 1);
+  std::vector<uint64_t> off(count + 1);
   std::vector<uint8_t> logs(count);
   int logmax = 0;
   for (size_t f = 0; f < count; ++f) {
@@ -452,33 +471,40 @@ int cyc_prefix_ifft_segmented(fp12* v, const int* log_sizes, const uint32_t* out
     logmax = std::max(logmax, log_sizes[f]);
     off[f + 1] = off[f] + ((uint64_t)1 << log_sizes[f]);
   }
-  for (size_t f = 0; f <= count; ++f) hoff[f] = off[f] / 2;
+  const uint64_t total = off[count];
+  if (total > 0xffffffffull) return set_error(BTE_EINVAL, "prefix_segmented: more than 2^32 elements");
+  std::vector<uint32_t> order;
+  std::vector<size_t> start(logmax + 1, 0);
+  for (int lh = 0; lh < logmax; ++lh) {
+    start[lh] = order.size();
+    stage_order(off, logs, out_counts, lh, order);
+  }
+  start[logmax] = order.size();
   DomainTables* t;
   int rc = domain_tables(logmax, true, true, &t, s);
   if (rc) return rc;
-  uint64_t *d_off, *d_hoff;
+  uint64_t* d_off;
   uint8_t* d_logs;
-  uint32_t* d_outs;
+  uint32_t* d_order;
   BTE_HIP(hipMallocAsync((void**)&d_off, (count + 1) * 8, s));
-  BTE_HIP(hipMallocAsync((void**)&d_hoff, (count + 1) * 8, s));
   BTE_HIP(hipMallocAsync((void**)&d_logs, count, s));
-  BTE_HIP(hipMallocAsync((void**)&d_outs, count * 4, s));
+  BTE_HIP(hipMallocAsync((void**)&d_order, std::max<size_t>(order.size(), 1) * 4, s));
   BTE_HIP(hipMemcpyAsync(d_off, off.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
-  BTE_HIP(hipMemcpyAsync(d_hoff, hoff.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
   BTE_HIP(hipMemcpyAsync(d_logs, logs.data(), count, hipMemcpyHostToDevice, s));
-  BTE_HIP(hipMemcpyAsync(d_outs, out_counts, count * 4, hipMemcpyHostToDevice, s));
+  if (!order.empty()) BTE_HIP(hipMemcpyAsync(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
   BTE_HIP(hipStreamSynchronize(s));
-  const uint64_t total = off[count];
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse_seg<fp12>), dim3(blocks_for(total, 256)), dim3(256), 0, s, v,
                      (const uint64_t*)d_off, (const uint8_t*)d_logs, (uint32_t)count, total);
-  for (int lh = 0; lh < logmax; ++lh)
-    hipLaunchKernelGGL(k_cyc_prefix_stage_seg, dim3(blocks_for(total / 2, 64)), dim3(64), 0, s, v,
-                       (const uint64_t*)d_off, (const uint64_t*)d_hoff, (const uint8_t*)d_logs,
-                       (const uint32_t*)d_outs, (uint32_t)count, total / 2, logmax, lh, (const cyc_plan*)t->plans);
+  for (int lh = 0; lh < logmax; ++lh) {
+    const uint64_t nbf = start[lh + 1] - start[lh];
+    if (nbf)
+      hipLaunchKernelGGL(k_cyc_stage_ord, dim3(blocks_for(nbf, 64)), dim3(64), 0, s, v,
+                         (const uint32_t*)d_order + start[lh], nbf, (const uint64_t*)d_off, (const uint8_t*)d_logs,
+                         (uint32_t)count, logmax, lh, (const cyc_plan*)t->plans);
+  }
   (void)hipFreeAsync(d_off, s);
-  (void)hipFreeAsync(d_hoff, s);
   (void)hipFreeAsync(d_logs, s);
-  (void)hipFreeAsync(d_outs, s);
+  (void)hipFreeAsync(d_order, s);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
@@ -569,6 +595,14 @@ extern "C" int bte_cyclotomic_pow_small(bte_fp12* values, const int64_t* coeffic
 
 // `count` transforms of sizes 2^log_sizes[f] (host array) stored back to back, all stages of
 // all of them in one launch per stage level.
+extern "C" int bte_ifft_cyclotomic_prefix_unscaled_segmented(bte_fp12* values, const int* log_sizes,
+                                                             const uint32_t* output_counts, size_t count,
+                                                             void* stream) {
+  if (count == 0) return BTE_OK;
+  if (!values || !log_sizes || !output_counts) return set_error(BTE_EINVAL, "prefix_segmented: null pointer");
+  return cyc_prefix_ifft_segmented((fp12*)values, log_sizes, output_counts, count, (hipStream_t)stream);
+}
+
 extern "C" int bte_fft_g1_segmented(bte_p1* values, const int* log_sizes, size_t count, int inverse, int scaled,
                                     void* stream) {
   return group_fft_segmented<fp>((jac<fp>*)values, log_sizes, count, inverse != 0, scaled != 0, (hipStream_t)stream);

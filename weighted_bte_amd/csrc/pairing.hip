@@ -113,16 +113,9 @@ __global__ void __launch_bounds__(64) k_prepare_lines(const g2a* __restrict__ q,
   }
 }
 
-__global__ void __launch_bounds__(64) k_miller_prepared(const line_t* __restrict__ lines, size_t ntables,
-                                                        const g1a* __restrict__ p, size_t n, fp12* __restrict__ out) {
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const line_t* L = lines + (i % ntables) * MILLER_LINES;
-  const g1a P = p[i];
-  if (aff_is_identity(P) || fp2_is_zero(L[0].cy)) {
-    out[i] = fp12_one();
-    return;
-  }
+// Miller loop of P against a prepared table (all-zero table = identity Q), conjugated.
+BTE_DEV fp12 miller_with_lines(const line_t* __restrict__ L, const g1a& P) {
+  if (aff_is_identity(P) || fp2_is_zero(L[0].cy)) return fp12_one();
   fp12 f = fp12_one();
   int k = 0;
 #pragma unroll 1
@@ -131,7 +124,33 @@ __global__ void __launch_bounds__(64) k_miller_prepared(const line_t* __restrict
     f = apply_line(f, L[k++], P);
     if ((MILLER_X_ABS >> b) & 1) f = apply_line(f, L[k++], P);
   }
-  out[i] = fp12_conj(f);
+  return fp12_conj(f);
+}
+
+__global__ void __launch_bounds__(64) k_miller_prepared(const line_t* __restrict__ lines, size_t ntables,
+                                                        const g1a* __restrict__ p, size_t n, fp12* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = miller_with_lines(lines + (i % ntables) * MILLER_LINES, p[i]);
+}
+
+__global__ void __launch_bounds__(64) k_miller_prepared_seg(const line_t* const* __restrict__ tables,
+                                                            const uint64_t* __restrict__ seg_off, uint32_t nseg,
+                                                            const g1a* __restrict__ p, size_t n,
+                                                            fp12* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = seg_of(seg_off, nseg, i);
+  out[i] = miller_with_lines(tables[s] + (i - seg_off[s]) * MILLER_LINES, p[i]);
+}
+
+int miller_prepared_segmented(const void* const* tables, const uint64_t* seg_off, size_t nseg, const void* p,
+                              size_t n, void* out, hipStream_t s) {
+  if (n == 0) return BTE_OK;
+  hipLaunchKernelGGL(k_miller_prepared_seg, dim3(blocks_for(n, 64)), dim3(64), 0, s, (const line_t* const*)tables,
+                     seg_off, (uint32_t)nseg, (const g1a*)p, n, (fp12*)out);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
 }
 
 __global__ void __launch_bounds__(64) k_miller_pairs(const g1a* __restrict__ p, const g2a* __restrict__ q, size_t n,
@@ -227,6 +246,15 @@ extern "C" int bte_miller_loop_prepared_tiled(const bte_fp6* lines, size_t ntabl
                      (const line_t*)lines, ntables, (const g1a*)p, n, (fp12*)out);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
+}
+
+extern "C" int bte_miller_loop_prepared_segmented(const bte_fp6* const* tables, const uint64_t* seg_offsets,
+                                                  size_t nseg, const bte_p1_affine* p, size_t n, bte_fp12* out,
+                                                  void* stream) {
+  if (n == 0) return BTE_OK;
+  if (!tables || !seg_offsets || !p || !out || nseg == 0)
+    return set_error(BTE_EINVAL, "miller_loop_prepared_segmented: bad arguments");
+  return miller_prepared_segmented((const void* const*)tables, seg_offsets, nseg, p, n, out, (hipStream_t)stream);
 }
 
 extern "C" int bte_miller_loop_pairs(const bte_p1_affine* p, const bte_p2_affine* q, size_t n, bte_fp12* out,

@@ -2,12 +2,14 @@
 
 Requests are batches of weighted_btx ciphertexts with the accepted shares of one committee.
 The server keeps each committee's key material and prepared state (kernel line tables and
-opening keys, per padded batch size) resident in HBM.  Every scheduling round it admits the
-waiting requests that fit the HBM budget, groups the admitted ones by (committee, padded batch
-size) and runs each group through one fused precompute + open.  A request of B ciphertexts is
-padded to the next power of two with invalid slots (identity first component), which the
-scheme opens to None; the padding is dropped from the result.  Requests that arrive while a
-round runs join the next round.
+opening keys, per padded batch size) resident in HBM.  Whenever requests are waiting and HBM
+budget and a launch slot are free, it admits the waiting requests that fit and runs all of them,
+whatever their committees and batch sizes, through one fused precompute + open
+(bte_btx_decrypt_requests): every stage is one launch over every admitted request.  A request
+of B ciphertexts runs against the state prepared for the next power of two, the extra slots
+being empty (the reference's None).  Up to `max_inflight` such launches overlap on the device,
+each on its own stream; requests that arrive while others run are admitted as soon as budget
+and a slot free up, and completion is observed with stream events.
 
 This is host control flow (the reference has no service layer); all arithmetic runs in the
 library's kernels.
@@ -44,6 +46,7 @@ class Request:
     admitted: float = 0.0
     done: float = 0.0
     messages: torch.Tensor = None
+    valid: list = None         # B flags (None: all valid)
 
     def footprint(self):
         """HBM bytes held while in flight: padded inputs, Miller/FFT scratch and outputs."""
@@ -70,25 +73,32 @@ class Committee:
 
 
 class BtxServer:
-    def __init__(self, committees, hbm_budget=8 << 30, max_fuse=128):
+    def __init__(self, committees, hbm_budget=8 << 30, max_fuse=256, max_inflight=2):
         self.committees = committees
         self.budget = hbm_budget
         self.max_fuse = max_fuse
+        self.max_inflight = max_inflight
+        self.streams = [torch.cuda.Stream() for _ in range(max_inflight)]
         self.waiting = collections.deque()
+        self.inflight = []   # (requests, event, bytes, stream index)
         self.finished = []
+        self.launches = 0
 
     def submit(self, req: Request):
         if next_pow2(req.batch) > self.committees[req.committee].max_batch:
             raise ValueError("BatchTooLarge")
         self.waiting.append(req)
 
+    def _used(self):
+        return sum(b for _, _, b, _ in self.inflight)
+
     def _admit(self):
-        admitted, used = [], 0
+        admitted, used = [], self._used()
         keep = collections.deque()
         while self.waiting:
             r = self.waiting.popleft()
             need = r.footprint()
-            if admitted and used + need > self.budget:
+            if len(admitted) >= self.max_fuse or ((admitted or self.inflight) and used + need > self.budget):
                 keep.append(r)
                 continue
             admitted.append(r)
@@ -96,48 +106,71 @@ class BtxServer:
         self.waiting = keep
         return admitted
 
-    def _run_group(self, com, bp, reqs):
-        lines, keys = com.prepare(bp)
-        T = len(com.accepted)
-        ct1 = torch.zeros(len(reqs) * bp * G1A, dtype=torch.uint8, device="cuda")
-        ct2 = torch.zeros(len(reqs) * bp * F12, dtype=torch.uint8, device="cuda")
-        valid = []
-        for k, r in enumerate(reqs):
-            ct1[k * bp * G1A:k * bp * G1A + r.batch * G1A] = r.ct1
-            ct2[k * bp * F12:k * bp * F12 + r.batch * F12] = r.ct2
-            valid += [True] * r.batch + [False] * (bp - r.batch)
-        shares = torch.cat([r.shares for r in reqs])
-        beta = api.btx_precompute_batched(ct1, valid, lines, bp)
-        out = api.btx_open_batched(shares, T, keys, ct2, beta, valid, bp)
-        for k, r in enumerate(reqs):
-            r.messages = out[k * bp * F12:k * bp * F12 + r.batch * F12]
+    def _launch(self, reqs, stream):
+        work = []
+        for r in reqs:
+            com = self.committees[r.committee]
+            bp = next_pow2(r.batch)
+            lines, keys = com.prepare(bp)
+            work.append({"ct1": r.ct1, "ct2": r.ct2, "valid": r.valid or [True] * r.batch, "batch_size": bp,
+                         "lines": lines, "keys": keys, "shares": r.shares, "n_accepted": len(com.accepted)})
+        stream.wait_stream(torch.cuda.current_stream())  # inputs and prepared state
+        with torch.cuda.stream(stream):
+            outs = api.btx_decrypt_requests(work)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        for r, o in zip(reqs, outs):
+            r.messages = o
+        return ev
 
-    def step(self, now_fn=time.perf_counter):
-        """One scheduling round: admit, group, run; returns the requests completed."""
-        admitted = self._admit()
-        if not admitted:
-            return []
-        t_adm = now_fn()
-        groups = collections.OrderedDict()
-        for r in admitted:
-            r.admitted = t_adm
-            groups.setdefault((r.committee, next_pow2(r.batch)), []).append(r)
-        for (cid, bp), reqs in groups.items():
-            for i in range(0, len(reqs), self.max_fuse):
-                self._run_group(self.committees[cid], bp, reqs[i:i + self.max_fuse])
-        torch.cuda.synchronize()
-        t = now_fn()
-        for r in admitted:
-            r.done = t
-        self.finished += admitted
-        return admitted
+    def poll(self, now_fn=time.perf_counter):
+        """Retire finished launches; returns the requests completed."""
+        done, keep = [], []
+        for reqs, ev, b, si in self.inflight:
+            if ev.query():
+                t = now_fn()
+                for r in reqs:
+                    r.done = t
+                done += reqs
+            else:
+                keep.append((reqs, ev, b, si))
+        self.inflight = keep
+        self.finished += done
+        return done
+
+    def step(self, now_fn=time.perf_counter, block=False):
+        """Retire finished work, then launch the admissible waiting requests if a slot is free.
+        With block=True waits for everything launched.  Returns the requests completed."""
+        done = self.poll(now_fn)
+        if self.waiting and len(self.inflight) < self.max_inflight:
+            admitted = self._admit()
+            if admitted:
+                busy = {si for _, _, _, si in self.inflight}
+                si = next(i for i in range(self.max_inflight) if i not in busy)
+                t = now_fn()
+                for r in admitted:
+                    r.admitted = t
+                ev = self._launch(admitted, self.streams[si])
+                self.launches += 1
+                self.inflight.append((admitted, ev, sum(r.footprint() for r in admitted), si))
+        if block:
+            for _, ev, _, _ in self.inflight:
+                ev.synchronize()
+            done += self.poll(now_fn)
+        return done
+
+    def drain(self, now_fn=time.perf_counter):
+        done = []
+        while self.waiting or self.inflight:
+            done += self.step(now_fn, block=True)
+        return done
 
     def decrypt_alone(self, req: Request):
         """The same request through the plain per-call path (no fusion, no padding)."""
         com = self.committees[req.committee]
         lines, keys = api.btx_prepare(com.material, com.total_weight, com.max_batch, com.offsets,
                                       com.accepted, req.batch)
-        valid = [True] * req.batch
+        valid = req.valid or [True] * req.batch
         beta = api.btx_precompute(req.ct1, valid, lines)
         return api.btx_open(req.shares, keys, req.ct2, beta, valid)
 
@@ -153,19 +186,19 @@ def run_poisson(server, requests, rate, now_fn=time.perf_counter, sleep_fn=time.
     for _ in requests:
         t += rnd.expovariate(rate)
         arrivals.append(t)
-    pending = list(zip(arrivals, requests))
+    pending = collections.deque(zip(arrivals, requests))
     done = 0
     while done < len(requests):
         now = now_fn() - t0
         while pending and pending[0][0] <= now:
-            at, r = pending.pop(0)
+            at, r = pending.popleft()
             r.arrival = t0 + at
             server.submit(r)
-        if not server.waiting:
-            if pending:
-                sleep_fn(max(0.0, pending[0][0] - (now_fn() - t0)))
-            continue
         done += len(server.step(now_fn))
+        if not server.waiting and not server.inflight and pending:
+            sleep_fn(max(0.0, pending[0][0] - (now_fn() - t0)))
+        elif server.inflight:
+            sleep_fn(0.0005)
     wall = now_fn() - t0
     lat = sorted(r.done - r.arrival for r in requests)
     p50 = lat[len(lat) // 2]
