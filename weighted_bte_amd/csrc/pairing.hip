@@ -186,8 +186,7 @@ __global__ void __launch_bounds__(64) k_segment_product(const fp12* __restrict__
 // Easy part over chunks of EASY_CHUNK elements sharing one Fp12 inversion (Montgomery's trick,
 // as batch_easy_final_exponentiation does for the whole batch; the values are identical).
 static constexpr int EASY_CHUNK = 4;
-__global__ void __launch_bounds__(64) k_final_exp_easy(const fp12* __restrict__ in, size_t n, fp12* __restrict__ out,
-                                                       int also_hard) {
+__global__ void __launch_bounds__(64) k_final_exp_easy(const fp12* __restrict__ in, size_t n, fp12* __restrict__ out) {
   const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   const size_t start = t * EASY_CHUNK;
   if (start >= n) return;
@@ -203,12 +202,12 @@ __global__ void __launch_bounds__(64) k_final_exp_easy(const fp12* __restrict__ 
     const fp12 f = in[i];
     const fp12 fi = fp12_mul(inv, out[i]);
     inv = fp12_mul(inv, f);
-    fp12 e = easy_part_with_inverse(f, fi);
-    out[i] = also_hard ? hard_part(e) : e;
+    out[i] = easy_part_with_inverse(f, fi);
   }
 }
 
-__global__ void __launch_bounds__(64) k_final_exp_hard(const fp12* __restrict__ in, size_t n, fp12* __restrict__ out) {
+// in == out is allowed (each thread reads its element before writing it).
+__global__ void __launch_bounds__(64) k_final_exp_hard(const fp12* in, size_t n, fp12* out) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   out[i] = hard_part(in[i]);
@@ -290,7 +289,7 @@ extern "C" int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, v
   if (!in || !out || (const void*)in == (const void*)out)
     return set_error(BTE_EINVAL, "final_exp_easy: null or aliased buffers");
   hipLaunchKernelGGL(k_final_exp_easy, dim3(blocks_for((n + EASY_CHUNK - 1) / EASY_CHUNK, 64)), dim3(64), 0,
-                     (hipStream_t)stream, (const fp12*)in, n, (fp12*)out, 0);
+                     (hipStream_t)stream, (const fp12*)in, n, (fp12*)out);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
@@ -304,12 +303,16 @@ extern "C" int bte_final_exp_hard(const bte_fp12* in, size_t n, bte_fp12* out, v
   return BTE_OK;
 }
 
+// The easy part shares an inversion over chunks of EASY_CHUNK elements (one thread per chunk);
+// the hard part then runs one thread per element, so every SIMD gets work at any n.
 extern "C" int bte_final_exp(const bte_fp12* in, size_t n, bte_fp12* out, void* stream) {
   if (n == 0) return BTE_OK;
   if (!in || !out || (const void*)in == (const void*)out)
     return set_error(BTE_EINVAL, "final_exp: null or aliased buffers");
   hipLaunchKernelGGL(k_final_exp_easy, dim3(blocks_for((n + EASY_CHUNK - 1) / EASY_CHUNK, 64)), dim3(64), 0,
-                     (hipStream_t)stream, (const fp12*)in, n, (fp12*)out, 1);
+                     (hipStream_t)stream, (const fp12*)in, n, (fp12*)out);
+  hipLaunchKernelGGL(k_final_exp_hard, dim3(blocks_for(n, 64)), dim3(64), 0, (hipStream_t)stream, (const fp12*)out,
+                     n, (fp12*)out);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
