@@ -70,6 +70,41 @@ def cpu_baseline(logn: int):
     return ocpu.time_g1_msm_sample(lib, logn)
 
 
+def bench_msm_sizes(tune=True, reps=5):
+    """Standalone MSM sizes G1 2^12, 2^16, 2^20 and G2 2^12, 2^16: closed-form parity, plain
+    per-call time (built-in window) and autotuned time, inputs resident in HBM."""
+    import torch
+
+    from weighted_bte_amd import api, autotune, inputs
+
+    rows = []
+    for group, logn in ((1, 12), (1, 16), (1, 20), (2, 12), (2, 16)):
+        n = 1 << logn
+        st = inputs.Stream(2 + group, 2000 + logn)
+        ks = st.scalars(n)
+        sc = inputs.msm_scalars(st, n)
+        gen = api.g1_generator_mul if group == 1 else api.g2_generator_mul
+        norm = api.batch_normalize_g1 if group == 1 else api.batch_normalize_g2
+        msm = api.g1_multi_exp_affine_bytes if group == 1 else api.g2_multi_exp_affine_bytes
+        bases = gen(api.to_device(inputs.pack_scalars(ks)))
+        dsc = api.to_device(inputs.pack_scalars(sc))
+        want = api.to_host(gen(api.to_device((sum(k * s for k, s in zip(ks, sc)) % R).to_bytes(32, "little"))))
+        ok = api.to_host(norm(msm(bases, dsc))) == want
+        row = {"group": "G%d" % group, "n": n, "default_window_bits": autotune.default_window(n),
+               "plain_ms": round(autotune.time_msm(group, bases, dsc, reps), 4)}
+        if tune:
+            best, sweep = autotune.tune_msm(group, bases, dsc)
+            ok = ok and api.to_host(norm(msm(bases, dsc))) == want
+            row.update({"tuned_window_bits": best, "tuned_ms": round(autotune.time_msm(group, bases, dsc, reps), 4),
+                        "sweep_ms": sweep})
+            api.set_msm_window(group, n, 0)
+        row["parity"] = bool(ok)
+        rows.append(row)
+        del bases, dsc
+        torch.cuda.empty_cache()
+    return rows
+
+
 def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batches=2, with_cpu=True):
     """weighted_btx batch decryption at B = 512 with n = 128 Pareto(1.16) weights scaled to
     W = 4096 (threshold W/2, heaviest-first committee).  Reports ctxts/s for the plain per-call
@@ -109,9 +144,11 @@ def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batche
     _, t_partial1 = timed(lambda: shares_for(1))
     shares = shares_for(batches)
 
-    prep = lambda: api.btx_prepare(wl.material, wl.W, wl.L, wl.offsets, committee, B)
+    def prep():
+        lines, keys = api.btx_prepare(wl.material, wl.W, wl.L, wl.offsets, committee, B)
+        return lines, keys, api.prepare_g2_lines(keys)  # opening keys' line tables stay resident
     prep()
-    (lines, keys), t_prepare = timed(prep)
+    (lines, keys, key_lines), t_prepare = timed(prep)
 
     def plain(nb):
         outs = []
@@ -124,7 +161,7 @@ def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batche
 
     def fused():
         beta = api.btx_precompute_batched(ct1, valid, lines, B)
-        return api.btx_open_batched(shares, T, keys, ct2, beta, valid, B)
+        return api.btx_open_prepared_batched(shares, T, key_lines, ct2, beta, valid, B)
 
     plain(1)
     out_plain, t_plain = timed(lambda: plain(plain_batches))
@@ -135,7 +172,7 @@ def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batche
     W.profile_enable(True)
     beta = api.btx_precompute_batched(ct1, valid, lines, B)
     ph_pre = W.profile_read()
-    api.btx_open_batched(shares, T, keys, ct2, beta, valid, B)
+    api.btx_open_prepared_batched(shares, T, key_lines, ct2, beta, valid, B)
     ph_open = W.profile_read()
     W.profile_enable(False)
     n = B * batches
@@ -225,6 +262,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--logn", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="time the built-in window table only")
+    ap.add_argument("--no-config2", action="store_true", help="skip the G1/G2 MSM size table")
     ap.add_argument("--serving-requests", type=int, default=48,
                     help="requests in the serving measurement (0: skip)")
     ap.add_argument("--decrypt-batches", type=int, default=128,
@@ -261,24 +300,38 @@ def main():
     if not parity:
         print("PARITY FAILURE: MSM result differs from the closed form", file=sys.stderr)
 
-    for _ in range(a.warmup):
-        api.g1_multi_exp_affine_bytes(bases, dev_sc)
-    torch.cuda.synchronize()
+    def timed_steps():
+        for _ in range(a.warmup):
+            api.g1_multi_exp_affine_bytes(bases, dev_sc)
+        torch.cuda.synchronize()
+        W.profile_enable(True)
+        ph = {}
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            api.g1_multi_exp_affine_bytes(bases, dev_sc)
+            for k, v in W.profile_read().items():
+                ph[k] = ph.get(k, 0.0) + v
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        W.profile_enable(False)
+        return (t1 - t0) * 1e3 / a.steps, {k: v / a.steps for k, v in ph.items()}
 
-    W.profile_enable(True)
-    phases = {}
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        api.g1_multi_exp_affine_bytes(bases, dev_sc)
-        for k, v in W.profile_read().items():
-            phases[k] = phases.get(k, 0.0) + v
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    W.profile_enable(False)
-    ms = (t1 - t0) * 1e3 / a.steps
-    phases = {k: v / a.steps for k, v in phases.items()}
-
+    # plain per-call kernels (built-in window table), then the autotuned window
+    ms_default, phases_default = timed_steps()
     c = msm_window_bits(n)
+    tuned = None
+    if not a.no_tune:
+        from weighted_bte_amd import autotune
+        best, sweep = autotune.tune_msm(1, bases, dev_sc)
+        got = api.to_host(api.batch_normalize_g1(api.g1_multi_exp_affine_bytes(bases, dev_sc)))
+        parity = parity and got == want
+        c = best
+        tuned = {"window_bits": best, "sweep_ms": sweep, "default_window_bits": msm_window_bits(n),
+                 "default_ms": round(ms_default, 4)}
+        ms, phases = timed_steps()
+    else:
+        ms, phases = ms_default, phases_default
+
     E = nonzero_signed_digits(sc_b, c)
     acc_macs = E * FP_MUL_PER_MIXED_ADD * MACS_PER_FP_MUL
     acc_ms = phases.get("accumulate", float("nan"))
@@ -286,6 +339,7 @@ def main():
     nw = (256 + c - 1) // c
     msm_fp_mul = nw * (11 * n + 32 * (1 << c)) + 7 * 255
     mac_peak = W.probe_mac_rate(torch.cuda.current_stream().cuda_stream)
+    config2 = None if a.no_config2 else bench_msm_sizes(not a.no_tune)
 
     cpu = None if a.no_cpu else cpu_baseline(a.logn)
     decrypt = bench_decrypt(a.decrypt_batches, with_cpu=not a.no_cpu) if a.decrypt_batches > 0 else None
@@ -306,6 +360,7 @@ def main():
         "data": "synthetic (seeded ChaCha20; bases k_i*G, 10% sparse scalars)",
         "config": {"workload": "g1_msm", "n": n, "window_bits": c, "sparse_fraction": 0.1,
                    "parallelism": "single GPU"},
+        "autotuned": tuned,
         "parity": "closed-form ok" if parity else "MISMATCH",
         "msm_per_s": round(1e3 / ms, 3),
         "phases_ms": {k: round(v, 4) for k, v in phases.items()},
@@ -323,11 +378,13 @@ def main():
             "mixed_adds_per_launch": E,
         },
         "cpu_baseline": cpu,
+        "msm_sizes": config2,
         "decrypt": decrypt,
         "serving": serving,
     }
     print(json.dumps(line))
     ok = parity and (decrypt is None or decrypt["plaintexts_bit_exact"]) and \
+        (config2 is None or all(r["parity"] for r in config2)) and \
         (serving is None or serving["plaintexts_bit_exact"])
     sys.exit(0 if ok else 1)
 

@@ -159,6 +159,12 @@ int bte_miller_loop_pairs(const bte_p1_affine* p, const bte_p2_affine* q, size_t
  * offsets, seg_offsets[nseg] = npairs); pairs with an identity are skipped, empty = one. */
 int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine* q, size_t npairs,
                           const uint64_t* seg_offsets, size_t nseg, bte_fp12* out, void* stream);
+/* unprepared_multi_miller_loop with the G2 side prepared: pair g uses the line table
+ * pair_tables[g] (DEVICE array of npairs pointers to 68-line tables) and p[g]; seg_offsets is a
+ * HOST array (nseg+1, from 0 to npairs).  The pairs of a segment share the accumulator and its
+ * squarings in chunks of up to 8 pairs.  Identity pairs are skipped, empty = one. */
+int bte_multi_miller_loop_prepared(const bte_fp6* const* pair_tables, const bte_p1_affine* p, size_t npairs,
+                                   const uint64_t* seg_offsets, size_t nseg, bte_fp12* out, void* stream);
 /* final_exponentiation.rs:276-305 easy_final_exponentiation / batch_easy_final_exponentiation
  * (inputs nonzero; in and out must not alias) */
 int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
@@ -220,16 +226,26 @@ typedef struct {
   size_t batch_size;
   const bte_fp6* kernel_lines;
   const bte_p2_affine* opening_keys;
+  const bte_fp6* opening_lines; /* NULL, or the keys' line tables (bte_g2_prepare_lines) */
   const bte_p1* shares;
   size_t n_accepted;
   bte_fp12* messages_out;
 } bte_btx_request;
 
 /* decryption.rs:555-603 precompute_batch followed by :607-695 open_batch for requests of
- * different committees and batch sizes at once: every stage (segmented G1 FFT, prepared Miller
+ * different committees and batch sizes at once (opening keys prepared when every request gives
+ * opening_lines): every stage (segmented G1 FFT, prepared Miller
  * loops, easy part, segmented prefix IFFT, hard part, uneven multi-Miller loops, final
  * exponentiation, combine) is one launch over all requests. */
 int bte_btx_decrypt_requests(const bte_btx_request* requests, size_t count, void* stream);
+
+/* open_batch with the opening keys prepared once per committee and batch size:
+ * key_lines = bte_g2_prepare_lines over the B x T opening keys (68 tables each, slot-major).
+ * The same messages as bte_btx_open_batched; the prepared multi-Miller loop shares its
+ * accumulator over chunks of pairs. */
+int bte_btx_open_prepared_batched(const bte_p1* shares, size_t n_accepted, const bte_fp6* key_lines,
+                                  const bte_fp12* ct2, const bte_fp12* beta, const uint8_t* valid,
+                                  size_t batch_size, size_t count, bte_fp12* messages_out, void* stream);
 
 /* decryption.rs:205-233 partial_decrypt for N parties at once: sigma_j = sum_i q_j^i ct1_i,
  * i = 1..B.  q = N canonical scalars (HOST); sigma_out = N G1 Jacobian (device). */

@@ -6,7 +6,7 @@ import random
 import pytest
 
 from btx_setup import BtxSetup, heaviest_first_committee
-from helpers import O
+from helpers import O, g1_mul_gen, g2_mul_gen
 
 pytestmark = pytest.mark.gpu
 
@@ -121,3 +121,65 @@ def test_btx_fused_batches_equal_alone(api):
     one = O.f12_to_bytes(O.F12_ONE)
     for i in range(B * K):
         assert fused[576 * i:576 * (i + 1)] == (want[576 * i:576 * (i + 1)] if valid[i] else one)
+
+
+def test_btx_open_prepared_equals_unprepared(api):
+    """Opening against the keys' line tables (all slots valid, and with invalid slots) gives the
+    same messages as the unprepared multi-Miller opening."""
+    from weighted_bte_amd import workloads
+
+    weights = [2, 3, 1, 4, 2, 5]
+    wl = workloads.BtxWorkload(weights, threshold=8, max_batch=8, seed=13)
+    committee = workloads.heaviest_first(weights, wl.t)
+    T, B, K = len(committee), 8, 2
+    ct1, ct2, msgs = wl.encrypt(B, K)
+    lines, keys = api.btx_prepare(wl.material, wl.W, wl.L, wl.offsets, committee, B)
+    key_lines = api.prepare_g2_lines(keys)
+    sh = []
+    for b in range(K):
+        sig = api.btx_partial_decrypt(ct1[b * B * 96:(b + 1) * B * 96], wl.q)
+        sh.append(b"".join(api.to_host(sig[144 * j:144 * (j + 1)]) for j in committee))
+    sh = api.to_device(b"".join(sh))
+    results = []
+    for valid in ([True] * (B * K), [i % 5 != 2 for i in range(B * K)]):
+        beta = api.btx_precompute_batched(ct1, valid, lines, B)
+        a = api.to_host(api.btx_open_batched(sh, T, keys, ct2, beta, valid, B))
+        p = api.to_host(api.btx_open_prepared_batched(sh, T, key_lines, ct2, beta, valid, B))
+        assert a == p
+        results.append(p)
+    # with every slot valid the shares match the batch, so the plaintexts come back
+    assert results[0] == api.to_host(msgs)
+    one = O.f12_to_bytes(O.F12_ONE)
+    assert all(results[1][576 * i:576 * (i + 1)] == one for i in range(B * K) if i % 5 == 2)
+
+
+def test_multi_miller_prepared_uneven_segments(api):
+    """Prepared multi-Miller loops over uneven segments (0, 1, 3, 9, 17 pairs, with identity
+    points) equal the unprepared ones bit for bit (same lines, products reordered)."""
+    import ctypes
+
+    import torch
+
+    from weighted_bte_amd import call
+
+    rnd = random.Random(77)
+    arities = [0, 1, 3, 9, 17]
+    n = sum(arities)
+    ps = [g1_mul_gen(rnd.randrange(1, O.R)) for _ in range(n)]
+    qs = [g2_mul_gen(rnd.randrange(1, O.R)) for _ in range(n)]
+    ps[4] = None  # identity G1 point in segment 3
+    qs[20] = None  # identity G2 point in segment 4
+    P = api.to_device(b"".join(O.g1_affine_to_bytes(x) for x in ps))
+    Q = api.to_device(b"".join(O.g2_affine_to_bytes(x) for x in qs))
+    off = [0]
+    for a in arities:
+        off.append(off[-1] + a)
+    want = api.unprepared_multi_miller_loop(P, Q, off)
+    lines = api.prepare_g2_lines(Q)
+    base = lines.data_ptr()
+    tabs = torch.tensor([base + 19584 * g for g in range(n)], dtype=torch.int64).cuda()
+    out = torch.empty(576 * len(arities), dtype=torch.uint8, device="cuda")
+    hoff = (ctypes.c_uint64 * len(off))(*off)
+    call("bte_multi_miller_loop_prepared", ctypes.c_void_p(tabs.data_ptr()), ctypes.c_void_p(P.data_ptr()), n,
+         hoff, len(arities), ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert api.to_host(out) == api.to_host(want)

@@ -220,3 +220,27 @@ def test_g2_msm_closed_form_2_16(api):
     out = api.g2_multi_exp_affine_bytes(bases, api.to_device(pack_scalars(sc)))
     want = g2_mul_gen(sum(k * s for k, s in zip(ks, sc)))
     assert O.g2_compress(g2_jac_affine(api.to_host(out))) == O.g2_compress(want)
+
+
+def test_autotune_window_keeps_result(api, tmp_path):
+    """The autotuner sweeps windows, installs the fastest and the MSM result is unchanged; its
+    table round-trips through save/apply."""
+    from weighted_bte_amd import autotune
+
+    n = 1 << 12
+    rng = O.SeededRng(9, 12)
+    ks = [rng.scalar() for _ in range(n)]
+    sc = O.msm_config_scalars(rng, n)
+    bases = api.g1_generator_mul(api.to_device(pack_scalars(ks)))
+    dsc = api.to_device(pack_scalars(sc))
+    want = O.g1_compress(g1_mul_gen(sum(k * s for k, s in zip(ks, sc))))
+    try:
+        best, times = autotune.tune_msm(1, bases, dsc, candidates=[6, 8, 10], reps=1)
+        assert set(times) == {6, 8, 10} and best in times
+        assert O.g1_compress(g1_jac_affine(api.to_host(api.g1_multi_exp_affine_bytes(bases, dsc)))) == want
+        path = str(tmp_path / "w.json")
+        autotune.save_table({("g1", 12): 7}, path)
+        assert autotune.apply_table(path) == 1
+        assert O.g1_compress(g1_jac_affine(api.to_host(api.g1_multi_exp_affine_bytes(bases, dsc)))) == want
+    finally:
+        api.set_msm_window(1, n, 0)

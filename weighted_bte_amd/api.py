@@ -406,7 +406,7 @@ class _BtxRequest(ctypes.Structure):
     _fields_ = [("ct1", ctypes.c_void_p), ("ct2", ctypes.c_void_p), ("valid", ctypes.c_void_p),
                 ("count", ctypes.c_size_t), ("batch_size", ctypes.c_size_t),
                 ("kernel_lines", ctypes.c_void_p), ("opening_keys", ctypes.c_void_p),
-                ("shares", ctypes.c_void_p), ("n_accepted", ctypes.c_size_t),
+                ("opening_lines", ctypes.c_void_p), ("shares", ctypes.c_void_p), ("n_accepted", ctypes.c_size_t),
                 ("messages_out", ctypes.c_void_p)]
 
 
@@ -414,7 +414,8 @@ def btx_decrypt_requests(requests):
     """precompute_batch + open_batch (decryption.rs:555-695) for requests of different committees
     and batch sizes in one fused pass.  Each request is a dict with ct1 (count G1 affine), ct2
     (count Fp12), valid (count bools), batch_size (the size its lines/keys were prepared for),
-    lines, keys, shares (T G1 Jacobian) and n_accepted.  Returns one message tensor per request."""
+    lines, keys, optionally key_lines (the keys' line tables), shares (T G1 Jacobian) and
+    n_accepted.  Returns one message tensor per request."""
     arr = (_BtxRequest * len(requests))()
     keep, outs = [], []
     for k, r in enumerate(requests):
@@ -424,8 +425,10 @@ def btx_decrypt_requests(requests):
         v = (ctypes.c_uint8 * max(n, 1))(*[1 if x else 0 for x in r["valid"]])
         keep.append(v)
         outs.append(out)
+        kl = r.get("key_lines")
         arr[k] = _BtxRequest(_ptr(_dev(r["ct1"])), _ptr(ct2), ctypes.cast(v, ctypes.c_void_p), n,
                              r["batch_size"], _ptr(_dev(r["lines"])), _ptr(_dev(r["keys"])),
+                             _ptr(_dev(kl)) if kl is not None else ctypes.c_void_p(0),
                              _ptr(_dev(r["shares"])), r["n_accepted"], _ptr(out))
     call("bte_btx_decrypt_requests", ctypes.cast(arr, ctypes.c_void_p), len(requests), _stream())
     return outs
@@ -445,6 +448,18 @@ def ifft_cyclotomic_prefix_unscaled_segmented(values, log_sizes, output_counts):
         parts.append(v[off * FP12:(off + c) * FP12])
         off += 1 << l
     return torch.cat(parts) if parts else v[:0]
+
+
+def btx_open_prepared_batched(shares_jac, n_accepted, key_lines, ct2, beta, valid, batch_size):
+    """open_batch against the opening keys' line tables (prepare_g2_lines(keys))."""
+    ct2 = _dev(ct2)
+    n = ct2.numel() // FP12
+    count = n // batch_size
+    v = (ctypes.c_uint8 * n)(*[1 if x else 0 for x in valid])
+    out = _empty(n * FP12, ct2)
+    call("bte_btx_open_prepared_batched", _ptr(_dev(shares_jac)), n_accepted, _ptr(_dev(key_lines)), _ptr(ct2),
+         _ptr(_dev(beta)), v, batch_size, count, _ptr(out), _stream())
+    return out
 
 
 def btx_partial_decrypt(ct1_affine, q_scalars):

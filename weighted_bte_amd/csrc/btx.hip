@@ -164,6 +164,15 @@ __global__ void k_open_pairs(const g1a* __restrict__ shares, const g2a* __restri
   Q[g] = keys[i * T + p];
 }
 
+__global__ void k_gather_pairs(const g1a* __restrict__ P, const bte_fp6* const* __restrict__ tab,
+                               const uint32_t* __restrict__ src, uint64_t n, g1a* __restrict__ P2,
+                               const bte_fp6** __restrict__ tab2) {
+  const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  P2[g] = P[src[g]];
+  tab2[g] = tab[src[g]];
+}
+
 template <class T> __global__ void k_tile(const T* __restrict__ in, size_t n, size_t reps, T* __restrict__ out) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i < n * reps) out[i] = in[i % n];
@@ -192,11 +201,14 @@ __global__ void __launch_bounds__(64) k_open_combine(const fp12* __restrict__ ct
 struct ReqDev {
   const g1a* ct1;
   const fp12* ct2;
-  const g2a* keys;  // batch_size x T, slot-major
+  const g2a* keys;         // batch_size x T, slot-major
+  const bte_fp6* klines;   // their line tables (68 per key) or null
   const g1j* shares;
   fp12* out;
   uint64_t count, T;
 };
+
+static constexpr size_t LINES_PER_TABLE = 68;
 
 // Request k owns m-slots [m_off[k], m_off[k+1]) of the padded FFT input: its ciphertexts' first
 // components (identity for invalid ones), then identities up to nextpow2(2 batch_size).
@@ -233,14 +245,28 @@ __global__ void k_req_gather_shares(const ReqDev* __restrict__ R, const uint64_t
 __global__ void k_req_pairs(const ReqDev* __restrict__ R, const uint64_t* __restrict__ slot_off,
                             const uint64_t* __restrict__ pair_off, const uint64_t* __restrict__ share_off,
                             uint32_t K, uint32_t S, uint64_t npairs, const g1a* __restrict__ sa,
-                            g1a* __restrict__ P, g2a* __restrict__ Q) {
+                            g1a* __restrict__ P, g2a* __restrict__ Q, const bte_fp6** __restrict__ tab) {
   const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   if (g >= npairs) return;
   const uint32_t j = seg_of(pair_off, S, g);
   const uint32_t k = seg_of(slot_off, K, j);
   const uint64_t p = g - pair_off[j], i = j - slot_off[k];
   P[g] = sa[share_off[k] + p];
-  Q[g] = R[k].keys[i * R[k].T + p];
+  if (tab)
+    tab[g] = R[k].klines + (i * R[k].T + p) * LINES_PER_TABLE;
+  else
+    Q[g] = R[k].keys[i * R[k].T + p];
+}
+
+// Prepared opening: pair g of batch b, slot i, party p: P = shares[b T + p] and the line table
+// of opening key (i, p).
+__global__ void k_open_pairs_prepared(const g1a* __restrict__ shares, const bte_fp6* __restrict__ key_lines, size_t T,
+                                      size_t B, size_t count, g1a* __restrict__ P, const bte_fp6** __restrict__ tab) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= T * B * count) return;
+  const size_t p = g % T, i = (g / T) % B, b = g / (T * B);
+  P[g] = shares[b * T + p];
+  tab[g] = key_lines + (i * T + p) * LINES_PER_TABLE;
 }
 
 __global__ void __launch_bounds__(64) k_req_combine(const ReqDev* __restrict__ R, const uint64_t* __restrict__ slot_off,
@@ -535,6 +561,7 @@ extern "C" int bte_btx_decrypt_requests(const bte_btx_request* requests, size_t 
   std::vector<uint32_t> outs;
   std::vector<uint64_t> m_off{0}, slot_off{0}, share_off{0}, pair_off{0};
   std::vector<uint8_t> valid;
+  bool all_lines = true;
   for (size_t k = 0; k < count; ++k) {
     const bte_btx_request& q = requests[k];
     if (q.count == 0) continue;
@@ -545,8 +572,9 @@ extern "C" int bte_btx_decrypt_requests(const bte_btx_request* requests, size_t 
     const size_t m = next_pow2(2 * q.batch_size);
     int lg = 0;
     while (((size_t)1 << lg) < m) ++lg;
-    R.push_back({(const g1a*)q.ct1, (const fp12*)q.ct2, (const g2a*)q.opening_keys, (const g1j*)q.shares,
-                 (fp12*)q.messages_out, q.count, q.n_accepted});
+    R.push_back({(const g1a*)q.ct1, (const fp12*)q.ct2, (const g2a*)q.opening_keys, q.opening_lines,
+                 (const g1j*)q.shares, (fp12*)q.messages_out, q.count, q.n_accepted});
+    all_lines = all_lines && q.opening_lines != nullptr;
     tables.push_back(q.kernel_lines);
     logs.push_back(lg);
     outs.push_back((uint32_t)q.count);
@@ -570,7 +598,8 @@ extern "C" int bte_btx_decrypt_requests(const bte_btx_request* requests, size_t 
   uint8_t* d_valid;
   g1j *c, *shj;
   g1a *ca, *sa, *P;
-  g2a* Q;
+  g2a* Q = nullptr;
+  const bte_fp6** tab = nullptr;
   fp12 *ml, *conv, *pref, *beta, *mm, *alpha;
   BTE_TRY(sc.upload(&d_R, R.data(), K));
   BTE_TRY(sc.upload(&d_tables, tables.data(), K));
@@ -588,7 +617,10 @@ extern "C" int bte_btx_decrypt_requests(const bte_btx_request* requests, size_t 
   BTE_TRY(sc.get(&shj, SH));
   BTE_TRY(sc.get(&sa, SH));
   BTE_TRY(sc.get(&P, NP));
-  BTE_TRY(sc.get(&Q, NP));
+  if (all_lines)
+    BTE_TRY(sc.get(&tab, NP));
+  else
+    BTE_TRY(sc.get(&Q, NP));
   BTE_TRY(sc.get(&mm, S));
   BTE_TRY(sc.get(&alpha, S));
   BTE_TRY(sc.uploads_done());
@@ -615,14 +647,91 @@ extern "C" int bte_btx_decrypt_requests(const bte_btx_request* requests, size_t 
   if (NP)
     hipLaunchKernelGGL(k_req_pairs, dim3(blocks_for(NP, 256)), dim3(256), 0, s, (const ReqDev*)d_R,
                        (const uint64_t*)d_soff, (const uint64_t*)d_poff, (const uint64_t*)d_shoff, (uint32_t)K,
-                       (uint32_t)S, NP, (const g1a*)sa, P, Q);
-  BTE_TRY(bte_multi_miller_loop((const bte_p1_affine*)P, (const bte_p2_affine*)Q, NP, d_poff, S, (bte_fp12*)mm, s));
+                       (uint32_t)S, NP, (const g1a*)sa, P, Q, tab);
+  if (all_lines)
+    BTE_TRY(multi_miller_prepared((const void* const*)tab, P, NP, pair_off.data(), S, mm, s));
+  else
+    BTE_TRY(bte_multi_miller_loop((const bte_p1_affine*)P, (const bte_p2_affine*)Q, NP, d_poff, S, (bte_fp12*)mm, s));
   pt.mark("multi_miller");
   BTE_TRY(bte_final_exp((const bte_fp12*)mm, S, (bte_fp12*)alpha, s));
   pt.mark("final_exp");
   hipLaunchKernelGGL(k_req_combine, dim3(blocks_for(S, 64)), dim3(64), 0, s, (const ReqDev*)d_R,
                      (const uint64_t*)d_soff, (uint32_t)K, S, (const uint8_t*)d_valid, (const fp12*)alpha,
                      (const fp12*)beta);
+  pt.mark("combine");
+  BTE_CHECK_LAUNCH();
+  if (pt.on) {
+    BTE_HIP(hipStreamSynchronize(s));
+    pt.publish();
+  }
+  return BTE_OK;
+}
+
+// open_batch with the opening keys' line tables (bte_g2_prepare_lines over the B x T keys,
+// committee state kept resident): same outputs as bte_btx_open_batched; invalid slots run no
+// pairs.
+extern "C" int bte_btx_open_prepared_batched(const bte_p1* shares, size_t n_accepted, const bte_fp6* key_lines,
+                                             const bte_fp12* ct2, const bte_fp12* beta, const uint8_t* valid,
+                                             size_t batch_size, size_t count, bte_fp12* messages_out,
+                                             void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t B = batch_size, T = n_accepted, K = count;
+  if (!shares || !key_lines || !ct2 || !beta || !valid || !messages_out)
+    return set_error(BTE_EINVAL, "btx_open_prepared: null pointer");
+  if (B == 0 || T == 0) return set_error(BTE_EINVAL, "btx_open_prepared: empty batch or committee");
+  if (K == 0) return BTE_OK;
+  const size_t npairs = K * B * T;
+  Scratch sc(s);
+  g1a *sa, *P;
+  const bte_fp6** tab;
+  fp12 *ml, *alpha;
+  uint8_t* d_valid;
+  BTE_TRY(sc.get(&sa, K * T));
+  BTE_TRY(sc.get(&P, npairs));
+  BTE_TRY(sc.get(&tab, npairs));
+  BTE_TRY(sc.get(&ml, K * B));
+  BTE_TRY(sc.get(&alpha, K * B));
+  BTE_TRY(sc.upload(&d_valid, valid, K * B));
+  BTE_TRY(sc.uploads_done());
+  // slot j owns pairs [j T, j T + T) when valid, none otherwise (its message is one)
+  std::vector<uint64_t> off(K * B + 1);
+  off[0] = 0;
+  for (size_t j = 0; j < K * B; ++j) off[j + 1] = off[j] + (valid[j] ? T : 0);
+  PhaseTimer pt(s);
+  pt.begin();
+  BTE_TRY(bte_g1_batch_normalize(shares, K * T, (bte_p1_affine*)sa, s));
+  hipLaunchKernelGGL(k_open_pairs_prepared, dim3(blocks_for(npairs, 256)), dim3(256), 0, s, (const g1a*)sa,
+                     key_lines, T, B, K, P, tab);
+  std::vector<uint64_t> seg(K * B + 1);
+  bool all_valid = true;
+  for (size_t j = 0; j < K * B; ++j) all_valid = all_valid && valid[j];
+  if (all_valid) {
+    for (size_t j = 0; j <= K * B; ++j) seg[j] = j * T;
+    BTE_TRY(multi_miller_prepared((const void* const*)tab, P, npairs, seg.data(), K * B, ml, s));
+  } else {
+    // the pairs of valid slots, gathered densely (segment j = off[j] .. off[j+1])
+    std::vector<uint32_t> src;
+    src.reserve(off[K * B]);
+    for (size_t j = 0; j < K * B; ++j)
+      if (valid[j])
+        for (size_t p = 0; p < T; ++p) src.push_back((uint32_t)(j * T + p));
+    g1a* P2;
+    const bte_fp6** tab2;
+    uint32_t* d_src;
+    BTE_TRY(sc.get(&P2, std::max<size_t>(src.size(), 1)));
+    BTE_TRY(sc.get(&tab2, std::max<size_t>(src.size(), 1)));
+    BTE_TRY(sc.upload(&d_src, src.data(), std::max<size_t>(src.size(), 1)));
+    BTE_TRY(sc.uploads_done());
+    if (!src.empty())
+      hipLaunchKernelGGL(k_gather_pairs, dim3(blocks_for(src.size(), 256)), dim3(256), 0, s, (const g1a*)P,
+                         (const bte_fp6* const*)tab, (const uint32_t*)d_src, (uint64_t)src.size(), P2, tab2);
+    BTE_TRY(multi_miller_prepared((const void* const*)tab2, P2, src.size(), off.data(), K * B, ml, s));
+  }
+  pt.mark("multi_miller");
+  BTE_TRY(bte_final_exp((const bte_fp12*)ml, K * B, (bte_fp12*)alpha, s));
+  pt.mark("final_exp");
+  hipLaunchKernelGGL(k_open_combine, dim3(blocks_for(K * B, 64)), dim3(64), 0, s, (const fp12*)ct2,
+                     (const fp12*)alpha, (const fp12*)beta, (const uint8_t*)d_valid, K * B, (fp12*)messages_out);
   pt.mark("combine");
   BTE_CHECK_LAUNCH();
   if (pt.on) {

@@ -55,6 +55,11 @@ int cyc_prefix_ifft_segmented(fp12* v, const int* log_sizes, const uint32_t* out
 int miller_prepared_segmented(const void* const* tables, const uint64_t* seg_off, size_t nseg, const void* p,
                               size_t n, void* out, hipStream_t s);
 
+// Multi-Miller loops over prepared tables (pair g: table pair_tables[g], point p[g]) with
+// HOST segment offsets; pairs of a segment share accumulators in chunks (pairing.hip).
+int multi_miller_prepared(const void* const* pair_tables, const void* p, size_t npairs, const uint64_t* seg_offsets,
+                          size_t nseg, void* out, hipStream_t s);
+
 // Segment holding flat index g: off[f] <= g < off[f + 1], off[0] = 0, count segments.
 __device__ inline uint32_t seg_of(const uint64_t* __restrict__ off, uint32_t count, uint64_t g) {
   uint32_t lo = 0, hi = count;

@@ -21,6 +21,9 @@
 // A multi-Miller loop over a segment of pairs equals the product of the per-pair loops (the
 // shared squaring distributes over the product), so pairs run one per thread and each segment
 // is multiplied together by one workgroup.
+#include <algorithm>
+#include <vector>
+
 #include "common.hpp"
 #include "curve.hpp"
 #include "cyclo.hpp"
@@ -160,6 +163,37 @@ __global__ void __launch_bounds__(64) k_miller_pairs(const g1a* __restrict__ p, 
   out[i] = miller_loop_unprepared(p[i], q[i]);
 }
 
+// Multi-Miller loop over prepared tables with a shared accumulator (the squarings of
+// blst_miller_loop_n shared by the pairs of a chunk): thread j runs pairs
+// [chunk_off[j], chunk_off[j+1]) of one segment, pair g against table tables[g] and point P[g];
+// pairs with an identity are skipped (final_exponentiation.rs:238-256).  out[j] is conjugated.
+__global__ void __launch_bounds__(64) k_multi_miller_prepared(const line_t* const* __restrict__ tables,
+                                                              const g1a* __restrict__ P,
+                                                              const uint64_t* __restrict__ chunk_off, size_t nchunks,
+                                                              fp12* __restrict__ out) {
+  const size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (j >= nchunks) return;
+  const uint64_t a = chunk_off[j], b = chunk_off[j + 1];
+  fp12 f = fp12_one();
+  int k = 0;
+#pragma unroll 1
+  for (int bit = 62; bit >= 0; --bit) {
+    if (bit != 62) f = fp12_sqr(f);
+    const int nsteps = ((MILLER_X_ABS >> bit) & 1) ? 2 : 1;
+#pragma unroll 1
+    for (int st = 0; st < nsteps; ++st, ++k) {
+#pragma unroll 1
+      for (uint64_t g = a; g < b; ++g) {
+        const line_t* L = tables[g];
+        const g1a Pg = P[g];
+        if (aff_is_identity(Pg) || fp2_is_zero(L[0].cy)) continue;
+        f = apply_line(f, L[k], Pg);
+      }
+    }
+  }
+  out[j] = fp12_conj(f);
+}
+
 // out[s] = prod of f[off[s] .. off[s+1]); one 64-thread workgroup per segment.
 __global__ void __launch_bounds__(64) k_segment_product(const fp12* __restrict__ f, const uint64_t* __restrict__ off,
                                                         size_t nseg, fp12* __restrict__ out) {
@@ -282,6 +316,50 @@ extern "C" int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine
   (void)hipFreeAsync(f, s);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
+}
+
+static constexpr uint64_t MM_CHUNK = 8;  // pairs sharing one accumulator
+
+int bte::multi_miller_prepared(const void* const* pair_tables, const void* p, size_t npairs, const uint64_t* seg_offsets,
+                          size_t nseg, void* out, hipStream_t s) {
+  if (nseg == 0) return BTE_OK;
+  std::vector<uint64_t> chunk_off{0}, seg_chunks{0};
+  for (size_t g = 0; g < nseg; ++g) {
+    const uint64_t a = seg_offsets[g], b = seg_offsets[g + 1];
+    if (b < a || b > npairs) return set_error(BTE_EINVAL, "multi_miller_prepared: bad segment offsets");
+    for (uint64_t c = a; c < b; c += MM_CHUNK) chunk_off.push_back(std::min(b, c + MM_CHUNK));
+    seg_chunks.push_back(chunk_off.size() - 1);
+  }
+  const size_t nchunks = chunk_off.size() - 1;
+  uint64_t *d_chunk, *d_segc;
+  fp12* part;
+  BTE_HIP(hipMallocAsync((void**)&d_chunk, chunk_off.size() * 8, s));
+  BTE_HIP(hipMallocAsync((void**)&d_segc, seg_chunks.size() * 8, s));
+  BTE_HIP(hipMallocAsync((void**)&part, std::max<size_t>(nchunks, 1) * sizeof(fp12), s));
+  BTE_HIP(hipMemcpyAsync(d_chunk, chunk_off.data(), chunk_off.size() * 8, hipMemcpyHostToDevice, s));
+  BTE_HIP(hipMemcpyAsync(d_segc, seg_chunks.data(), seg_chunks.size() * 8, hipMemcpyHostToDevice, s));
+  BTE_HIP(hipStreamSynchronize(s));  // host vectors are temporaries
+  if (nchunks)
+    hipLaunchKernelGGL(k_multi_miller_prepared, dim3(blocks_for(nchunks, 64)), dim3(64), 0, s,
+                       (const line_t* const*)pair_tables, (const g1a*)p, (const uint64_t*)d_chunk, nchunks, part);
+  hipLaunchKernelGGL(k_segment_product, dim3((unsigned)nseg), dim3(64), 0, s, (const fp12*)part,
+                     (const uint64_t*)d_segc, nseg, (fp12*)out);
+  (void)hipFreeAsync(d_chunk, s);
+  (void)hipFreeAsync(d_segc, s);
+  (void)hipFreeAsync(part, s);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+extern "C" int bte_multi_miller_loop_prepared(const bte_fp6* const* pair_tables, const bte_p1_affine* p, size_t npairs,
+                                              const uint64_t* seg_offsets, size_t nseg, bte_fp12* out, void* stream) {
+  if (nseg == 0) return BTE_OK;
+  if (!seg_offsets || !out || (npairs && (!p || !pair_tables)))
+    return set_error(BTE_EINVAL, "multi_miller_loop_prepared: null pointer");
+  if (seg_offsets[0] != 0 || seg_offsets[nseg] != npairs)
+    return set_error(BTE_EINVAL, "multi_miller_loop_prepared: offsets must run from 0 to npairs");
+  return multi_miller_prepared((const void* const*)pair_tables, p, npairs, seg_offsets, nseg, out,
+                               (hipStream_t)stream);
 }
 
 extern "C" int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, void* stream) {

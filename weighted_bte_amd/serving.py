@@ -1,8 +1,8 @@
 """Continuous-batching decryption service over the fused weighted_btx entry points.
 
 Requests are batches of weighted_btx ciphertexts with the accepted shares of one committee.
-The server keeps each committee's key material and prepared state (kernel line tables and
-opening keys, per padded batch size) resident in HBM.  Whenever requests are waiting and HBM
+The server keeps each committee's key material and prepared state (kernel line tables, opening
+keys and the opening keys' line tables, per padded batch size) resident in HBM.  Whenever requests are waiting and HBM
 budget and a launch slot are free, it admits the waiting requests that fit and runs all of them,
 whatever their committees and batch sizes, through one fused precompute + open
 (bte_btx_decrypt_requests): every stage is one launch over every admitted request.  A request
@@ -63,12 +63,13 @@ class Committee:
     max_batch: int
     offsets: list
     accepted: list
-    prepared: dict = field(default_factory=dict)  # padded B -> (lines, keys)
+    prepared: dict = field(default_factory=dict)  # padded B -> (lines, keys, key lines)
 
     def prepare(self, bp):
         if bp not in self.prepared:
-            self.prepared[bp] = api.btx_prepare(self.material, self.total_weight, self.max_batch,
-                                                self.offsets, self.accepted, bp)
+            lines, keys = api.btx_prepare(self.material, self.total_weight, self.max_batch, self.offsets,
+                                          self.accepted, bp)
+            self.prepared[bp] = (lines, keys, api.prepare_g2_lines(keys))
         return self.prepared[bp]
 
 
@@ -111,9 +112,10 @@ class BtxServer:
         for r in reqs:
             com = self.committees[r.committee]
             bp = next_pow2(r.batch)
-            lines, keys = com.prepare(bp)
+            lines, keys, key_lines = com.prepare(bp)
             work.append({"ct1": r.ct1, "ct2": r.ct2, "valid": r.valid or [True] * r.batch, "batch_size": bp,
-                         "lines": lines, "keys": keys, "shares": r.shares, "n_accepted": len(com.accepted)})
+                         "lines": lines, "keys": keys, "key_lines": key_lines, "shares": r.shares,
+                         "n_accepted": len(com.accepted)})
         stream.wait_stream(torch.cuda.current_stream())  # inputs and prepared state
         with torch.cuda.stream(stream):
             outs = api.btx_decrypt_requests(work)
