@@ -30,6 +30,25 @@ MACS_PER_FP_MUL = 300  # 12-limb Montgomery: 144 product + 144 reduction + 12 qu
 FP_MUL_PER_MIXED_ADD = 11  # madd-2007-bl: 7M + 4S
 
 
+COPY_BYTES = 256 << 20
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "g1_msm_2p20_hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary (FETCH_SIZE
+    and WRITE_SIZE runs, scaled by the copy-kernel calibration), or None."""
+    if not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    for line in open(TRAFFIC_SUMMARY):
+        parts = [p.strip() for p in line.strip().strip("|").split("|")]
+        if len(parts) >= 5 and parts[0] == kernel:
+            try:
+                return int(parts[4].replace(",", ""))
+            except ValueError:
+                return None
+    return None
+
+
 def msm_window_bits(n: int) -> int:
     lg = n.bit_length() - 1
     return max(4, min(16, lg - 4))
@@ -264,6 +283,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="time the built-in window table only")
     ap.add_argument("--no-config2", action="store_true", help="skip the G1/G2 MSM size table")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
     ap.add_argument("--serving-requests", type=int, default=48,
                     help="requests in the serving measurement (0: skip)")
     ap.add_argument("--decrypt-batches", type=int, default=128,
@@ -291,6 +312,13 @@ def main():
     bases = api.g1_generator_mul(dev_ks)
     torch.cuda.synchronize()
     print("inputs ready in %.1f s" % (time.time() - t_gen), file=sys.stderr, flush=True)
+
+    if a.copy_calibration:
+        src = torch.ones(COPY_BYTES, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # parity: MSM == (sum k_i s_i) * G, as affine bytes
     out = api.g1_multi_exp_affine_bytes(bases, dev_sc)
@@ -372,7 +400,9 @@ def main():
             "peak": round(LANE_OPS_PEAK / 1e12, 2),
             "unit": "TOP/s (u32 multiply-accumulates)",
             "frac": round(achieved / (LANE_OPS_PEAK / 1e12), 4),
-            "traffic": None,
+            "traffic": measured_traffic("k_msm_accumulate") if n == 1 << 20 else None,
+            "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "algorithmic_bytes_per_launch": E * (96 + 4),
             "measured_mac_peak": round(mac_peak / 1e12, 3),
             "frac_of_measured_mac_peak": round(achieved / (mac_peak / 1e12), 4),
             "mixed_adds_per_launch": E,
