@@ -151,6 +151,58 @@ template <class F> BTE_DEV jac<F> jac_mul_scalar(const jac<F>& p, const uint32_t
   return acc;
 }
 
+// Joint double-and-add for k1 P + k2 Q with k1, k2 below 2^128 (four u32 words each).
+template <class F>
+BTE_DEV jac<F> jac_mul2(const jac<F>& p, const jac<F>& q, const uint32_t* __restrict__ k1,
+                        const uint32_t* __restrict__ k2) {
+  const jac<F> pq = jac_add(p, q);
+  int top = 127;
+  while (top >= 0 && !(((k1[top >> 5] | k2[top >> 5]) >> (top & 31)) & 1u)) --top;
+  jac<F> acc = jac_identity<F>();
+#pragma unroll 1
+  for (int i = top; i >= 0; --i) {
+    acc = jac_dbl(acc);
+    const uint32_t b1 = (k1[i >> 5] >> (i & 31)) & 1u, b2 = (k2[i >> 5] >> (i & 31)) & 1u;
+    if (b1 | b2) acc = jac_add(acc, b1 ? (b2 ? pq : p) : q);
+  }
+  return acc;
+}
+
+// G1: phi(X, Y, Z) = (beta X, Y, Z) is multiplication by lambda = |z|^2 - 1, so for
+// k = k1 + k2 lambda (k[0..3] = k1, k[4..7] = k2, both below 2^128) k P = k1 P + k2 phi(P).
+BTE_DEV jac<fp> g1_glv_mul(const jac<fp>& p, const uint32_t* __restrict__ k) {
+  fp beta;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) beta.l[i] = G1_BETA[i];
+  const jac<fp> q = {fp_mul(p.x, beta), p.y, p.z};
+  return jac_mul2(p, q, k, k + 4);
+}
+
+// G2: psi(X, Y, Z) = (conj(X) c_x, conj(Y) c_y, conj(Z)) is multiplication by z, so psi^2 is
+// multiplication by |z|^2; for k = k1 + k2 |z|^2 (k[0..3], k[4..7]) k Q = k1 Q + k2 psi^2(Q).
+BTE_DEV jac<fp2> g2_psi(const jac<fp2>& p) {
+  fp2 cx, cy;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    cx.c0.l[i] = G2_PSI[0][0][i];
+    cx.c1.l[i] = G2_PSI[0][1][i];
+    cy.c0.l[i] = G2_PSI[1][0][i];
+    cy.c1.l[i] = G2_PSI[1][1][i];
+  }
+  return {fp2_mul(fp2_conj(p.x), cx), fp2_mul(fp2_conj(p.y), cy), fp2_conj(p.z)};
+}
+
+BTE_DEV jac<fp2> g2_gls_mul(const jac<fp2>& p, const uint32_t* __restrict__ k) {
+  return jac_mul2(p, g2_psi(g2_psi(p)), k, k + 4);
+}
+
+// Multiplication by an FFT twiddle from its precomputed two-part decomposition (8 x u32 for G1,
+// then 8 x u32 for G2: 16 words per twiddle).
+BTE_DEV jac<fp> twiddle_mul(const jac<fp>& p, const uint32_t* __restrict__ dec) { return g1_glv_mul(p, dec); }
+BTE_DEV jac<fp2> twiddle_mul(const jac<fp2>& p, const uint32_t* __restrict__ dec) {
+  return g2_gls_mul(p, dec + 8);
+}
+
 // Left-to-right double-and-add by a canonical scalar (8 x u32, below 2^255).
 template <class F> BTE_DEV jac<F> group_scalar_mul(const jac<F>& p, const uint32_t* __restrict__ k) {
   jac<F> acc = jac_identity<F>();

@@ -10,6 +10,9 @@
 //
 // Every stage is one launch with one thread per butterfly; twiddles w^j (j < n/2) and their
 // cyclotomic plans are built once per (size, direction) on the host and kept in device memory.
+// Group butterflies multiply by the twiddle through the curve endomorphisms: k = k1 + k2 lambda
+// on G1 (phi, lambda = |z|^2 - 1) and k = k1 + k2 |z|^2 on G2 (psi^2), a joint double-and-add of
+// at most 128 doublings instead of 255.
 // Butterflies are numbered twiddle-major: the lanes of a wavefront take butterflies that share
 // one twiddle (same exponent, so the same branch pattern of the double-and-add or NAF chain)
 // whenever a stage has at least 64 butterflies per twiddle, instead of 64 different exponents
@@ -85,7 +88,39 @@ static bteh::Fr domain_generator(int logn, bool inverse) {
   return inverse ? bteh::fr_inv(g) : g;
 }
 
+// k = q d + r for a canonical k (four u64 limbs) and 2^64 < d < 2^128; q and r must be below
+// 2^128 (true for k < r_Fr and the divisors used here, |z|^2 - 1 and |z|^2).
+static void divmod_u128(const uint64_t k[4], unsigned __int128 d, unsigned __int128& q, unsigned __int128& rem) {
+  q = 0;
+  rem = 0;
+  for (int i = 255; i >= 0; --i) {
+    const bool carry = (rem >> 127) != 0;
+    rem = (rem << 1) | ((k[i >> 6] >> (i & 63)) & 1);
+    if (carry || rem >= d) {
+      rem -= d;
+      q |= (unsigned __int128)1 << i;  // i < 128 whenever this fires (q < 2^128)
+    }
+  }
+}
+
+// Twiddle decompositions for the endomorphism-accelerated group butterflies: words 0..7 hold
+// (k mod lambda, k div lambda) with lambda = |z|^2 - 1 (G1, phi), words 8..15 hold
+// (k mod |z|^2, k div |z|^2) (G2, psi^2); each half as four little-endian u32.
+static void twiddle_decomposition(const bteh::Fr& k, uint32_t out[16]) {
+  const unsigned __int128 z2 = (unsigned __int128)X_ABS * X_ABS;
+  const unsigned __int128 divs[2] = {z2 - 1, z2};
+  for (int h = 0; h < 2; ++h) {
+    unsigned __int128 q, r;
+    divmod_u128(k.l, divs[h], q, r);
+    for (int i = 0; i < 4; ++i) {
+      out[8 * h + i] = (uint32_t)(r >> (32 * i));
+      out[8 * h + 4 + i] = (uint32_t)(q >> (32 * i));
+    }
+  }
+}
+
 struct DomainTables {
+  uint32_t* dec = nullptr;       // n/2 twiddle decompositions (16 x u32 each)
   uint32_t* twiddles = nullptr;  // n/2 canonical scalars (8 x u32 each): w^j
   cyc_plan* plans = nullptr;     // n/2 plans of w^j
   uint32_t* size_inv = nullptr;  // n^-1 (8 x u32)
@@ -109,6 +144,10 @@ static int domain_tables(int logn, bool inverse, bool want_plans, DomainTables**
     bteh::Fr ninv = bteh::fr_inv(bteh::fr_from_u64((uint64_t)1 << logn));
     BTE_HIP(hipMalloc((void**)&t.twiddles, tw.size() * 32));
     BTE_HIP(hipMemcpy(t.twiddles, tw.data(), tw.size() * 32, hipMemcpyHostToDevice));
+    std::vector<uint32_t> dec(tw.size() * 16);
+    for (size_t j = 0; j < tw.size(); ++j) twiddle_decomposition(tw[j], &dec[16 * j]);
+    BTE_HIP(hipMalloc((void**)&t.dec, dec.size() * 4));
+    BTE_HIP(hipMemcpy(t.dec, dec.data(), dec.size() * 4, hipMemcpyHostToDevice));
     BTE_HIP(hipMalloc((void**)&t.size_inv, 32));
     BTE_HIP(hipMemcpy(t.size_inv, ninv.l, 32, hipMemcpyHostToDevice));
   }
@@ -175,7 +214,7 @@ BTE_DEV size_t butterfly_slot(size_t g, size_t n, int log_half, size_t count, si
 // One DIT stage for butterfly size 2 * 2^log_half; twiddle index k << (logn - 1 - log_half).
 template <class F>
 __global__ void __launch_bounds__(64) k_group_stage(jac<F>* __restrict__ v, size_t n, int logn, int log_half,
-                                                    const uint32_t* __restrict__ tw, size_t count) {
+                                                    const uint32_t* __restrict__ dec, size_t count) {
   const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (g >= (n / 2) * count) return;
   const size_t half = (size_t)1 << log_half;
@@ -183,7 +222,7 @@ __global__ void __launch_bounds__(64) k_group_stage(jac<F>* __restrict__ v, size
   const size_t i = butterfly_slot(g, n, log_half, count, k);
   const jac<F> u = v[i];
   jac<F> w = v[i + half];
-  if (k) w = group_scalar_mul(w, tw + 8 * (k << (logn - 1 - log_half)));
+  if (k) w = twiddle_mul(w, dec + 16 * (k << (logn - 1 - log_half)));
   v[i] = jac_add(u, w);
   v[i + half] = jac_sub(u, w);
 }
@@ -259,7 +298,7 @@ static int group_fft(jac<F>* v, size_t n, size_t count, bool inverse, bool scale
                      logn, count);
   for (int lh = 0; lh < logn; ++lh)
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage<F>), dim3(blocks_for(n / 2 * count, 64)), dim3(64), 0, s, v, n,
-                       logn, lh, (const uint32_t*)t->twiddles, count);
+                       logn, lh, (const uint32_t*)t->dec, count);
   pt.mark("stages");
   if (inverse && scaled)
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_scale<F>), dim3(blocks_for(n * count, 64)), dim3(64), 0, s, v,
@@ -347,7 +386,7 @@ template <class F>
 __global__ void __launch_bounds__(64) k_group_stage_ord(jac<F>* __restrict__ v, const uint32_t* __restrict__ order,
                                                         uint64_t nbf, const uint64_t* __restrict__ off,
                                                         uint32_t count, int logmax, int log_half,
-                                                        const uint32_t* __restrict__ tw) {
+                                                        const uint32_t* __restrict__ dec) {
   const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   if (g >= nbf) return;
   const size_t i = order[g];
@@ -356,7 +395,7 @@ __global__ void __launch_bounds__(64) k_group_stage_ord(jac<F>* __restrict__ v, 
   const size_t k = (i - off[f]) & (half - 1);
   const jac<F> u = v[i];
   jac<F> w = v[i + half];
-  if (k) w = group_scalar_mul(w, tw + 8 * (k << (logmax - 1 - log_half)));
+  if (k) w = twiddle_mul(w, dec + 16 * (k << (logmax - 1 - log_half)));
   v[i] = jac_add(u, w);
   v[i + half] = jac_sub(u, w);
 }
@@ -419,7 +458,7 @@ static int group_fft_segmented(jac<F>* v, const int* log_sizes, size_t count, bo
     if (nbf)
       hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage_ord<F>), dim3(blocks_for(nbf, 64)), dim3(64), 0, s, v,
                          (const uint32_t*)d_order + start[lh], nbf, (const uint64_t*)d_off, (uint32_t)count, logmax,
-                         lh, (const uint32_t*)t->twiddles);
+                         lh, (const uint32_t*)t->dec);
   }
   if (inverse && scaled)
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_scale_seg<F>), dim3(blocks_for(total, 64)), dim3(64), 0, s, v,

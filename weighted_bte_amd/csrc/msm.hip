@@ -400,6 +400,27 @@ __global__ void k_msm_scatter_keys(const uint32_t* __restrict__ keys, size_t nte
   sorted[pos] = term | ((e & 1u) << 31);
 }
 
+// Per (segment, window) t: T_t = sum_j acc_j + L * sum_j j * run_j over its S reduction
+// segments (the bucket sum sum_k k B_k split at granularity L = 2^logL).
+template <class F>
+__global__ void __launch_bounds__(64) k_msm_seg_window(const jac<F>* __restrict__ seg_acc,
+                                                       const jac<F>* __restrict__ seg_run, size_t nsw, int logL,
+                                                       int logS, jac<F>* __restrict__ win) {
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= nsw) return;
+  const size_t S = (size_t)1 << logS;
+  const jac<F>* acc = seg_acc + t * S;
+  const jac<F>* run = seg_run + t * S;
+  jac<F> a = jac_identity<F>(), r = jac_identity<F>(), rs = jac_identity<F>();
+  for (size_t j = S; j-- > 1;) {
+    r = jac_add(r, run[j]);
+    rs = jac_add(rs, r);  // sum_j j * run_j
+  }
+  for (int k = 0; k < logL; ++k) rs = jac_dbl(rs);
+  for (size_t j = 0; j < S; ++j) a = jac_add(a, acc[j]);
+  win[t] = jac_add(a, rs);
+}
+
 template <class F>
 __global__ void __launch_bounds__(64) k_msm_combine(const jac<F>* __restrict__ win, uint32_t nseg, int nw, int c,
                                                     jac<F>* __restrict__ out) {
@@ -611,8 +632,11 @@ static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t
   std::lock_guard<std::mutex> ws_lock(ws->mu);
   Carver cv(s);
   uint32_t *keys, *counts, *offsets, *cursor, *sorted, *tile_sums, *big, *nbig;
-  jac<F> *buckets, *heads, *tails, *win, *runs;
+  jac<F> *buckets, *heads, *tails, *win, *seg_acc, *seg_run;
   int* err;
+  // bucket reduction in two levels: runs of L = 2^logL buckets per thread, then per window
+  const int logL = std::min(3, c - 1), logS = (c - 1) - logL;
+  const size_t nsw = (size_t)nseg * nw, nss = nsw << logS;
   cv.add(&keys, E_max);
   cv.add(&counts, nb);
   cv.add(&offsets, (size_t)nb + 1);
@@ -622,8 +646,9 @@ static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t
   cv.add(&buckets, nb);
   cv.add(&heads, nchunks);
   cv.add(&tails, nchunks);
-  cv.add(&win, (size_t)nseg * nw);
-  cv.add(&runs, (size_t)nseg * nw);
+  cv.add(&win, nsw);
+  cv.add(&seg_acc, nss);
+  cv.add(&seg_run, nss);
   cv.add(&err, 1);
   cv.add(&big, max_big);
   cv.add(&nbig, 1);
@@ -653,8 +678,10 @@ static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_big<F>), dim3((unsigned)max_big), dim3(64), 0, s,
                      (const uint32_t*)offsets, buckets, (const jac<F>*)heads, (const jac<F>*)tails,
                      (const uint32_t*)big, (const uint32_t*)nbig);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_segsum<F>), dim3(blocks_for((size_t)nseg * nw, 64)), dim3(64), 0, s,
-                     (const jac<F>*)buckets, (int)(nseg * nw), M, c - 1, 0, win, runs);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_segsum<F>), dim3(blocks_for(nss, 64)), dim3(64), 0, s,
+                     (const jac<F>*)buckets, (int)nsw, M, logL, logS, seg_acc, seg_run);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_seg_window<F>), dim3(blocks_for(nsw, 64)), dim3(64), 0, s,
+                     (const jac<F>*)seg_acc, (const jac<F>*)seg_run, nsw, logL, logS, win);
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_combine<F>), dim3(blocks_for(nseg, 64)), dim3(64), 0, s,
                      (const jac<F>*)win, (uint32_t)nseg, nw, c, out);
   BTE_CHECK_LAUNCH();
