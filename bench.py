@@ -25,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 R = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
+P_FIELD = 0x1A0111EA397FE69A4B1BA7B6434BACD764774B84F38512BF6730D2A0F6B0F6241EABFFFEB153FFFFB9FEFFFFFFFFAAAB
 LANE_OPS_PEAK = 256 * 4 * 16 * 2.4e9  # 32-bit VALU lane operations per second (MI355X, 2.4 GHz)
 MACS_PER_FP_MUL = 300  # 12-limb Montgomery: 144 product + 144 reduction + 12 quotient
 FP_MUL_PER_MIXED_ADD = 11  # madd-2007-bl: 7M + 4S
@@ -122,6 +123,143 @@ def bench_msm_sizes(tune=True, reps=5):
         del bases, dsc
         torch.cuda.empty_cache()
     return rows
+
+
+def _timed(fn, reps=3):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    best, out = float("inf"), None
+    for _ in range(reps):
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t)
+    return out, best * 1e3
+
+
+def bench_fft_sizes():
+    """Radix-2 group FFTs over the Fr domain: G1 2^8/2^12/2^16, G2 2^8/2^12, GT 2^8 (forward), and
+    64 G1 transforms of log-uniform sizes 2^6..2^12 in one launch sequence; inputs s_i * G, parity
+    = inverse(forward(x)) == x."""
+    import torch
+
+    from weighted_bte_amd import api, inputs
+
+    rows = []
+    for group, logn in ((1, 8), (1, 12), (1, 16), (2, 8), (2, 12), (3, 8)):
+        n = 1 << logn
+        st = inputs.Stream(7, 3000 + 10 * group + logn)
+        sb = api.to_device(inputs.pack_scalars(st.scalars(n)))
+        d = api.Radix2Domain(n)
+        if group == 3:
+            x = api.cyclotomic_mul_scalar_decomposed(api.to_device(_gt_bytes() * n), st.scalars(n))
+            fwd, inv, norm = d.fft_gt, d.ifft_gt, (lambda t: t)
+        else:
+            aff = (api.g1_generator_mul if group == 1 else api.g2_generator_mul)(sb)
+            x = _affine_to_jac(api, aff, group)
+            fwd = d.fft_g1 if group == 1 else d.fft_g2
+            inv = d.ifft_g1 if group == 1 else d.ifft_g2
+            norm = api.batch_normalize_g1 if group == 1 else api.batch_normalize_g2
+        y, ms = _timed(lambda: fwd(x))
+        ok = torch.equal(norm(inv(y)), norm(x))
+        rows.append({"group": ("G1", "G2", "GT")[group - 1], "n": n, "forward_ms": round(ms, 3),
+                     "butterflies_per_s": round((n // 2) * logn / (ms * 1e-3), 1), "roundtrip_ok": bool(ok)})
+    rng = __import__("random").Random(64)
+    logs = [6 + rng.randrange(7) for _ in range(64)]
+    total = sum(1 << l for l in logs)
+    st = inputs.Stream(7, 3999)
+    x = _affine_to_jac(api, api.g1_generator_mul(api.to_device(inputs.pack_scalars(st.scalars(total)))), 1)
+    y, ms = _timed(lambda: api.fft_g1_segmented(x, logs))
+    ok = torch.equal(api.batch_normalize_g1(api.fft_g1_segmented(y, logs, inverse=True)), api.batch_normalize_g1(x))
+    rows.append({"group": "G1", "batched": 64, "log_sizes": "6..12 log-uniform", "elements": total,
+                 "forward_ms": round(ms, 3), "roundtrip_ok": bool(ok)})
+    return rows
+
+
+def _gt_bytes():
+    from weighted_bte_amd import workloads
+    if not hasattr(_gt_bytes, "v"):
+        _gt_bytes.v = workloads.gt_generator_bytes()
+    return _gt_bytes.v
+
+
+def _affine_to_jac(api, aff, group):
+    """Affine device points (none the identity) -> Jacobian with Z = 1, built on the device."""
+    import torch
+    w = 96 if group == 1 else 192
+    n = aff.numel() // w
+    one = ((1 << 384) % P_FIELD).to_bytes(48, "little") + (bytes(48) if group == 2 else b"")
+    z = api.to_device(one).view(1, -1).expand(n, -1)
+    return torch.cat([aff.view(n, w), z], dim=1).reshape(-1).contiguous()
+
+
+def bench_multi_pairing(nslots=512, seed=11):
+    """Per-ciphertext multi-pairings of uneven arity (log-uniform in 2..512 pairs) in one fused
+    launch sequence with one shared batched final exponentiation: unprepared (G2 points) and
+    prepared (G2 line tables); parity against gT^(sum a_i b_i) for pairs (a_i G1, b_i G2)."""
+    import math
+    import random
+
+    import torch
+
+    from weighted_bte_amd import api, inputs
+
+    rnd = random.Random(seed)
+    ar = [int(round(math.exp(rnd.uniform(math.log(2), math.log(512))))) for _ in range(nslots)]
+    off = [0]
+    for a in ar:
+        off.append(off[-1] + a)
+    npairs = off[-1]
+    st = inputs.Stream(11, 4000)
+    a_s, b_s = st.scalars(npairs), st.scalars(npairs)
+    P = api.g1_generator_mul(api.to_device(inputs.pack_scalars(a_s)))
+    Q = api.g2_generator_mul(api.to_device(inputs.pack_scalars(b_s)))
+    un, ms_un = _timed(lambda: api.full_final_exponentiation(api.unprepared_multi_miller_loop(P, Q, off)))
+    lines, ms_lines = _timed(lambda: api.prepare_g2_lines(Q), reps=1)
+    pr, ms_pr = _timed(lambda: api.full_final_exponentiation(api.multi_miller_loop_prepared(lines, P, off)))
+    check = [0, 1, nslots // 2, nslots - 1]
+    exps = [sum(a * b for a, b in zip(a_s[off[s]:off[s + 1]], b_s[off[s]:off[s + 1]])) % R for s in check]
+    want = api.to_host(api.cyclotomic_mul_scalar_decomposed(api.to_device(_gt_bytes() * len(check)), exps))
+    got_un = b"".join(api.to_host(un[576 * s:576 * (s + 1)]) for s in check)
+    ok = got_un == want and torch.equal(un, pr)
+    return {"slots": nslots, "pairs": npairs, "arity": "log-uniform 2..512",
+            "unprepared_ms": round(ms_un, 3), "prepared_ms": round(ms_pr, 3),
+            "prepare_lines_ms": round(ms_lines, 3),
+            "pairs_per_s_unprepared": round(npairs / (ms_un * 1e-3), 1),
+            "pairs_per_s_prepared": round(npairs / (ms_pr * 1e-3), 1), "parity": bool(ok)}
+
+
+def bench_pfe(B=512, n_parties=128, total_weight=4096):
+    """weighted_pfe batch decryption at B = 512 (n = 128 Pareto(1.16) parties, W = 4096,
+    heaviest-first committee): prepare, precompute and open through the plain per-call entry
+    points; plaintexts checked."""
+    import torch
+
+    from weighted_bte_amd import api, workloads
+
+    t0 = time.time()
+    weights = workloads.pareto_weights(n_parties, total_weight)
+    wl = workloads.PfeWorkload(weights, total_weight // 2, B, seed=6)
+    committee = workloads.heaviest_first(weights, wl.t)
+    ct1, ct2, msgs = wl.encrypt()
+    sigma = api.pfe_partial_decrypt(ct1, wl.g)
+    shares = torch.cat([sigma[144 * j:144 * (j + 1)] for j in committee])
+    torch.cuda.synchronize()
+    t_setup = time.time() - t0
+    prep, ms_prep = _timed(lambda: api.pfe_prepare(wl.material, wl.W, B, wl.offsets, committee, wl.alpha,
+                                                   wl.global_key), reps=1)
+    d1p, tl, ml = prep
+
+    def dec():
+        fc, pc = api.pfe_precompute(ct1, tl)
+        return api.pfe_open(shares, d1p, ml, ct1, ct2, fc, pc, wl.alpha)
+    out, ms = _timed(dec)
+    return {"workload": "weighted_pfe decrypt", "B": B, "parties": n_parties, "total_weight": total_weight,
+            "committee": len(committee), "setup_s": round(t_setup, 1), "prepare_ms": round(ms_prep, 2),
+            "precompute_open_ms": round(ms, 2), "ctxts_per_s_cached_plain": round(B / (ms * 1e-3), 1),
+            "ctxts_per_s_cold_plain": round(B / ((ms + ms_prep) * 1e-3), 1),
+            "plaintexts_bit_exact": bool(torch.equal(out, msgs))}
 
 
 def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batches=2, with_cpu=True):
@@ -282,7 +420,9 @@ def main():
     ap.add_argument("--logn", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="time the built-in window table only")
-    ap.add_argument("--no-config2", action="store_true", help="skip the G1/G2 MSM size table")
+    ap.add_argument("--no-config2", action="store_true",
+                    help="skip the MSM size table, the FFT size table and the uneven multi-pairing")
+    ap.add_argument("--pfe", action="store_true", help="also measure weighted_pfe decryption at B = 512")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
     ap.add_argument("--serving-requests", type=int, default=48,
@@ -368,6 +508,9 @@ def main():
     msm_fp_mul = nw * (11 * n + 32 * (1 << c)) + 7 * 255
     mac_peak = W.probe_mac_rate(torch.cuda.current_stream().cuda_stream)
     config2 = None if a.no_config2 else bench_msm_sizes(not a.no_tune)
+    ffts = None if a.no_config2 else bench_fft_sizes()
+    mpair = None if a.no_config2 else bench_multi_pairing()
+    pfe = bench_pfe() if a.pfe else None
 
     cpu = None if a.no_cpu else cpu_baseline(a.logn)
     decrypt = bench_decrypt(a.decrypt_batches, with_cpu=not a.no_cpu) if a.decrypt_batches > 0 else None
@@ -409,12 +552,17 @@ def main():
         },
         "cpu_baseline": cpu,
         "msm_sizes": config2,
+        "fft_sizes": ffts,
+        "multi_pairing": mpair,
+        "pfe_decrypt": pfe,
         "decrypt": decrypt,
         "serving": serving,
     }
     print(json.dumps(line))
     ok = parity and (decrypt is None or decrypt["plaintexts_bit_exact"]) and \
         (config2 is None or all(r["parity"] for r in config2)) and \
+        (ffts is None or all(r["roundtrip_ok"] for r in ffts)) and \
+        (mpair is None or mpair["parity"]) and (pfe is None or pfe["plaintexts_bit_exact"]) and \
         (serving is None or serving["plaintexts_bit_exact"])
     sys.exit(0 if ok else 1)
 

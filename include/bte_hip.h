@@ -162,7 +162,8 @@ int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine* q, size_t
 /* unprepared_multi_miller_loop with the G2 side prepared: pair g uses the line table
  * pair_tables[g] (DEVICE array of npairs pointers to 68-line tables) and p[g]; seg_offsets is a
  * HOST array (nseg+1, from 0 to npairs).  The pairs of a segment share the accumulator and its
- * squarings in chunks of up to 8 pairs.  Identity pairs are skipped, empty = one. */
+ * squarings in chunks of up to 8 pairs (fewer when npairs < 2^19, to keep the GPU full).
+ * Identity pairs are skipped, empty = one. */
 int bte_multi_miller_loop_prepared(const bte_fp6* const* pair_tables, const bte_p1_affine* p, size_t npairs,
                                    const uint64_t* seg_offsets, size_t nseg, bte_fp12* out, void* stream);
 /* final_exponentiation.rs:276-305 easy_final_exponentiation / batch_easy_final_exponentiation

@@ -268,6 +268,21 @@ def prepare_g2_lines(q_affine):
     return out
 
 
+def multi_miller_loop_prepared(lines, p_affine, seg_offsets):
+    """unprepared_multi_miller_loop with the G2 side prepared: pair g uses table g of `lines`
+    (prepare_g2_lines of the G2 points) and point g; one Fp12 per segment of seg_offsets."""
+    lines, p = _dev(lines), _dev(p_affine)
+    n = p.numel() // G1_AFFINE
+    if lines.numel() != n * LINES_BYTES or seg_offsets[0] != 0 or seg_offsets[-1] != n:
+        raise ValueError("lines / offsets do not match the points")
+    tabs = torch.arange(n, dtype=torch.int64, device=p.device) * LINES_BYTES + lines.data_ptr()
+    nseg = len(seg_offsets) - 1
+    offs = (ctypes.c_uint64 * len(seg_offsets))(*seg_offsets)
+    out = _empty(nseg * FP12, p)
+    call("bte_multi_miller_loop_prepared", _ptr(tabs), _ptr(p), n, offs, nseg, _ptr(out), _stream())
+    return out
+
+
 def miller_loop_prepared(lines, p_affine):
     lines, p = _dev(lines), _dev(p_affine)
     n = p.numel() // G1_AFFINE

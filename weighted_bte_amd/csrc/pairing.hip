@@ -318,16 +318,19 @@ extern "C" int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine
   return BTE_OK;
 }
 
-static constexpr uint64_t MM_CHUNK = 8;  // pairs sharing one accumulator
+// Pairs sharing one accumulator: up to 8, fewer when that would leave fewer than 2^16 threads
+// (one wave per SIMD), since the shared squarings save less than idle SIMDs cost.
+static constexpr uint64_t MM_CHUNK_MAX = 8, MM_MIN_THREADS = 1 << 16;
 
 int bte::multi_miller_prepared(const void* const* pair_tables, const void* p, size_t npairs, const uint64_t* seg_offsets,
                           size_t nseg, void* out, hipStream_t s) {
   if (nseg == 0) return BTE_OK;
+  const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(MM_CHUNK_MAX, npairs / MM_MIN_THREADS));
   std::vector<uint64_t> chunk_off{0}, seg_chunks{0};
   for (size_t g = 0; g < nseg; ++g) {
     const uint64_t a = seg_offsets[g], b = seg_offsets[g + 1];
     if (b < a || b > npairs) return set_error(BTE_EINVAL, "multi_miller_prepared: bad segment offsets");
-    for (uint64_t c = a; c < b; c += MM_CHUNK) chunk_off.push_back(std::min(b, c + MM_CHUNK));
+    for (uint64_t c = a; c < b; c += chunk) chunk_off.push_back(std::min(b, c + chunk));
     seg_chunks.push_back(chunk_off.size() - 1);
   }
   const size_t nchunks = chunk_off.size() - 1;
