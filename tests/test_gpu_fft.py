@@ -251,3 +251,29 @@ def test_cyclotomic_prefix_ifft_segmented(api):
     got = read_f12(api, api.ifft_cyclotomic_prefix_unscaled_segmented(x, logs, counts), sum(counts))
     want = [v for seg, c in zip(segs, counts) for v in O.ifft_cyclotomic_prefix_unscaled(seg, c)[:c]]
     assert got == want
+
+
+def test_cyclotomic_prefix_one_lane_equals_three_lane(api):
+    """64 prefix transforms of size 1024 in one call (32768 butterflies per stage: one thread per
+    butterfly) equal the same transforms one call each (512 butterflies: three lanes per element),
+    bit for bit."""
+    import ctypes
+
+    import torch
+
+    from weighted_bte_amd import call, workloads
+
+    n, count, outs = 1024, 64, 512
+    rnd = random.Random(90)
+    gt = workloads.gt_generator_bytes()
+    x = api.cyclotomic_mul_scalar_decomposed(api.to_device(gt * (n * count)),
+                                             [rnd.randrange(O.R) for _ in range(n * count)])
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    a = x.clone()
+    call("bte_ifft_cyclotomic_prefix_unscaled_batched", ctypes.c_void_p(a.data_ptr()), n, count, outs, stream)
+    b = x.clone()
+    for f in range(count):
+        call("bte_ifft_cyclotomic_prefix_unscaled", ctypes.c_void_p(b.data_ptr() + f * n * 576), n, outs, stream)
+    a = a.view(count, n * 576)[:, :outs * 576]
+    b = b.view(count, n * 576)[:, :outs * 576]
+    assert torch.equal(a, b)

@@ -24,6 +24,7 @@
 
 #include "common.hpp"
 #include "curve.hpp"
+#include "coop12.hpp"
 #include "cyclo.hpp"
 #include "host_field.hpp"
 
@@ -234,6 +235,56 @@ __global__ void __launch_bounds__(64) k_group_scale(jac<F>* __restrict__ v, size
   v[i] = group_scalar_mul(v[i], s);
 }
 
+// Cyclotomic stages come in two forms.  One thread per butterfly (k_cyc_stage, ...) is the
+// faster per element once a stage has enough butterflies to fill the GPU; below that, each
+// element runs on three lanes (coop12.hpp, *3 kernels: 21 butterflies per 64-lane workgroup,
+// `tab` holding the per-lane odd powers of the plan exponentiation, 768 bytes per lane of the
+// grid), which shortens the per-stage latency of small transforms.  A stage with fewer than
+// C3_MAX_BUTTERFLIES butterflies takes the three-lane form.
+static constexpr int C3_PER_WG = 21;
+static constexpr size_t C3_MAX_BUTTERFLIES = 32768;
+
+BTE_DEV c3tab c3_tab_for(uint32_t* tab) {
+  return {tab, (size_t)gridDim.x * 64, blockIdx.x * (size_t)64 + (threadIdx.x & 63)};
+}
+
+__global__ void __launch_bounds__(64) k_cyc_stage3(fp12* __restrict__ v, size_t n, int logn, int log_half,
+                                                   const cyc_plan* __restrict__ plans, size_t count,
+                                                   uint32_t* __restrict__ tab) {
+  __shared__ uint32_t xbuf[48 * 64];
+  const c3ctx c = c3_make(xbuf);
+  const size_t g = blockIdx.x * (size_t)C3_PER_WG + c.lane / 3;
+  if (c.lane >= 3 * C3_PER_WG || g >= (n / 2) * count) return;
+  const size_t half = (size_t)1 << log_half;
+  size_t k;
+  const size_t i = butterfly_slot(g, n, log_half, count, k);
+  const fp4 u = c3_load(c, v + i);
+  fp4 w = c3_load(c, v + i + half);
+  if (k) w = c3_cyclo_pow_plan(c, c3_tab_for(tab), w, plans + (k << (logn - 1 - log_half)));
+  const fp4 lo = c3_mul(c, u, w);
+  const fp4 hi = c3_mul(c, u, c3_conj(c, w));
+  c3_store(c, v + i, lo);
+  c3_store(c, v + i + half, hi);
+}
+
+__global__ void __launch_bounds__(64) k_cyc_prefix_last3(fp12* __restrict__ v, size_t n, size_t count,
+                                                         const cyc_plan* __restrict__ plans, size_t ntrans,
+                                                         uint32_t* __restrict__ tab) {
+  __shared__ uint32_t xbuf[48 * 64];
+  const c3ctx c = c3_make(xbuf);
+  const size_t g = blockIdx.x * (size_t)C3_PER_WG + c.lane / 3;
+  if (c.lane >= 3 * C3_PER_WG || g >= count * ntrans) return;
+  const size_t k = g / ntrans;  // twiddle-major
+  v += (g - k * ntrans) * n;
+  const size_t half = n / 2;
+  fp4 w = c3_load(c, v + half + k);
+  if (k) w = c3_cyclo_pow_plan(c, c3_tab_for(tab), w, plans + k);
+  const fp4 out = c3_mul(c, c3_load(c, v + k), w);
+  c3_store(c, v + k, out);
+}
+
+static unsigned c3_blocks(size_t nbutterflies) { return (unsigned)((nbutterflies + C3_PER_WG - 1) / C3_PER_WG); }
+
 __global__ void __launch_bounds__(64) k_cyc_stage(fp12* __restrict__ v, size_t n, int logn, int log_half,
                                                   const cyc_plan* __restrict__ plans, size_t count) {
   const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -324,15 +375,30 @@ static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool sc
   DomainTables* t;
   int rc = domain_tables(logn, inverse, true, &t, s);
   if (rc) return rc;
+  const size_t nbf = n / 2 * count;
+  const bool three = nbf < C3_MAX_BUTTERFLIES;
+  uint32_t* tab = nullptr;  // odd powers of the plan exponentiations, 768 bytes per lane
+  if (three) BTE_HIP(hipMallocAsync((void**)&tab, (size_t)c3_blocks(nbf) * 64 * 768, s));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse<fp12>), dim3(blocks_for(n * count, 256)), dim3(256), 0, s, v, n,
                      logn, count);
   const int full_stages = mode == 1 ? logn - 1 : logn;
-  for (int lh = 0; lh < full_stages; ++lh)
-    hipLaunchKernelGGL(k_cyc_stage, dim3(blocks_for(n / 2 * count, 64)), dim3(64), 0, s, v, n, logn, lh,
-                       (const cyc_plan*)t->plans, count);
-  if (mode == 1 && out_count)
-    hipLaunchKernelGGL(k_cyc_prefix_last, dim3(blocks_for(out_count * count, 64)), dim3(64), 0, s, v, n, out_count,
-                       (const cyc_plan*)t->plans, count);
+  for (int lh = 0; lh < full_stages; ++lh) {
+    if (three)
+      hipLaunchKernelGGL(k_cyc_stage3, dim3(c3_blocks(nbf)), dim3(64), 0, s, v, n, logn, lh,
+                         (const cyc_plan*)t->plans, count, tab);
+    else
+      hipLaunchKernelGGL(k_cyc_stage, dim3(blocks_for(nbf, 64)), dim3(64), 0, s, v, n, logn, lh,
+                         (const cyc_plan*)t->plans, count);
+  }
+  if (mode == 1 && out_count) {
+    if (three)
+      hipLaunchKernelGGL(k_cyc_prefix_last3, dim3(c3_blocks(out_count * count)), dim3(64), 0, s, v, n, out_count,
+                         (const cyc_plan*)t->plans, count, tab);
+    else
+      hipLaunchKernelGGL(k_cyc_prefix_last, dim3(blocks_for(out_count * count, 64)), dim3(64), 0, s, v, n,
+                         out_count, (const cyc_plan*)t->plans, count);
+  }
+  if (tab) (void)hipFreeAsync(tab, s);
   if (mode == 0 && inverse && scaled)
     hipLaunchKernelGGL(k_cyc_scale, dim3(blocks_for(n * count, 64)), dim3(64), 0, s, v, n * count,
                        (const cyc_plan*)t->size_inv_plan);
@@ -475,6 +541,33 @@ static int group_fft_segmented(jac<F>* v, const int* log_sizes, size_t count, bo
 // sizes 2^logs[f] stored back to back, keeping outs[f] outputs of transform f.  At stage level
 // lh a transform with lh < logs[f]-1 runs a full stage and one with lh == logs[f]-1 runs its
 // truncated last stage; plans come from the inverse domain of the largest size.
+__global__ void __launch_bounds__(64) k_cyc_stage_ord3(fp12* __restrict__ v, const uint32_t* __restrict__ order,
+                                                       uint64_t nbf, const uint64_t* __restrict__ off,
+                                                       const uint8_t* __restrict__ logs, uint32_t count, int logmax,
+                                                       int log_half, const cyc_plan* __restrict__ plans,
+                                                       uint32_t* __restrict__ tab) {
+  __shared__ uint32_t xbuf[48 * 64];
+  const c3ctx c = c3_make(xbuf);
+  const uint64_t g = blockIdx.x * (uint64_t)C3_PER_WG + c.lane / 3;
+  if (c.lane >= 3 * C3_PER_WG || g >= nbf) return;
+  const size_t i = order[g];
+  const uint32_t f = seg_of(off, count, i);
+  const size_t half = (size_t)1 << log_half;
+  const size_t k = (i - off[f]) & (half - 1);
+  fp4 w = c3_load(c, v + i + half);
+  if (k) w = c3_cyclo_pow_plan(c, c3_tab_for(tab), w, plans + (k << (logmax - 1 - log_half)));
+  const fp4 u = c3_load(c, v + i);
+  if (logs[f] - 1 == log_half) {  // truncated last stage: left output only
+    const fp4 lo = c3_mul(c, u, w);
+    c3_store(c, v + i, lo);
+    return;
+  }
+  const fp4 lo = c3_mul(c, u, w);
+  const fp4 hi = c3_mul(c, u, c3_conj(c, w));
+  c3_store(c, v + i, lo);
+  c3_store(c, v + i + half, hi);
+}
+
 __global__ void __launch_bounds__(64) k_cyc_stage_ord(fp12* __restrict__ v, const uint32_t* __restrict__ order,
                                                       uint64_t nbf, const uint64_t* __restrict__ off,
                                                       const uint8_t* __restrict__ logs, uint32_t count, int logmax,
@@ -531,16 +624,28 @@ int cyc_prefix_ifft_segmented(fp12* v, const int* log_sizes, const uint32_t* out
   BTE_HIP(hipMemcpyAsync(d_off, off.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
   BTE_HIP(hipMemcpyAsync(d_logs, logs.data(), count, hipMemcpyHostToDevice, s));
   if (!order.empty()) BTE_HIP(hipMemcpyAsync(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
+  size_t widest3 = 0;  // widest stage that takes the three-lane form
+  for (int lh = 0; lh < logmax; ++lh) {
+    const size_t nbf = start[lh + 1] - start[lh];
+    if (nbf < C3_MAX_BUTTERFLIES) widest3 = std::max(widest3, nbf);
+  }
+  uint32_t* tab = nullptr;
+  if (widest3) BTE_HIP(hipMallocAsync((void**)&tab, (size_t)c3_blocks(widest3) * 64 * 768, s));
   BTE_HIP(hipStreamSynchronize(s));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse_seg<fp12>), dim3(blocks_for(total, 256)), dim3(256), 0, s, v,
                      (const uint64_t*)d_off, (const uint8_t*)d_logs, (uint32_t)count, total);
   for (int lh = 0; lh < logmax; ++lh) {
     const uint64_t nbf = start[lh + 1] - start[lh];
-    if (nbf)
+    if (nbf && nbf < C3_MAX_BUTTERFLIES)
+      hipLaunchKernelGGL(k_cyc_stage_ord3, dim3(c3_blocks(nbf)), dim3(64), 0, s, v,
+                         (const uint32_t*)d_order + start[lh], nbf, (const uint64_t*)d_off, (const uint8_t*)d_logs,
+                         (uint32_t)count, logmax, lh, (const cyc_plan*)t->plans, tab);
+    else if (nbf)
       hipLaunchKernelGGL(k_cyc_stage_ord, dim3(blocks_for(nbf, 64)), dim3(64), 0, s, v,
                          (const uint32_t*)d_order + start[lh], nbf, (const uint64_t*)d_off, (const uint8_t*)d_logs,
                          (uint32_t)count, logmax, lh, (const cyc_plan*)t->plans);
   }
+  if (tab) (void)hipFreeAsync(tab, s);
   (void)hipFreeAsync(d_off, s);
   (void)hipFreeAsync(d_logs, s);
   (void)hipFreeAsync(d_order, s);
