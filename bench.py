@@ -230,36 +230,67 @@ def bench_multi_pairing(nslots=512, seed=11):
             "pairs_per_s_prepared": round(npairs / (ms_pr * 1e-3), 1), "parity": bool(ok)}
 
 
-def bench_pfe(B=512, n_parties=128, total_weight=4096):
+def bench_pfe(B=512, n_parties=128, total_weight=4096, batches=128, plain_batches=2):
     """weighted_pfe batch decryption at B = 512 (n = 128 Pareto(1.16) parties, W = 4096,
-    heaviest-first committee): prepare, precompute and open through the plain per-call entry
-    points; plaintexts checked."""
+    heaviest-first committee): prepare, then precompute + open per batch through the plain
+    per-call entry points and for `batches` batches through the fused ones (d1 line tables
+    resident); plaintexts checked."""
     import torch
 
+    import weighted_bte_amd as Wm
     from weighted_bte_amd import api, workloads
 
     t0 = time.time()
     weights = workloads.pareto_weights(n_parties, total_weight)
     wl = workloads.PfeWorkload(weights, total_weight // 2, B, seed=6)
     committee = workloads.heaviest_first(weights, wl.t)
-    ct1, ct2, msgs = wl.encrypt()
-    sigma = api.pfe_partial_decrypt(ct1, wl.g)
-    shares = torch.cat([sigma[144 * j:144 * (j + 1)] for j in committee])
+    T = len(committee)
+    ct1, ct2, msgs = wl.encrypt(batches)
+    sh = []
+    for b in range(batches):
+        sigma = api.pfe_partial_decrypt(ct1[b * B * 96:(b + 1) * B * 96], wl.g)
+        sh.append(torch.cat([sigma[144 * j:144 * (j + 1)] for j in committee]))
+    shares = torch.cat(sh)
     torch.cuda.synchronize()
     t_setup = time.time() - t0
-    prep, ms_prep = _timed(lambda: api.pfe_prepare(wl.material, wl.W, B, wl.offsets, committee, wl.alpha,
-                                                   wl.global_key), reps=1)
-    d1p, tl, ml = prep
 
-    def dec():
-        fc, pc = api.pfe_precompute(ct1, tl)
-        return api.pfe_open(shares, d1p, ml, ct1, ct2, fc, pc, wl.alpha)
-    out, ms = _timed(dec)
+    def prep():
+        d1p, tl, ml = api.pfe_prepare(wl.material, wl.W, B, wl.offsets, committee, wl.alpha, wl.global_key)
+        return d1p, tl, ml, api.prepare_g2_lines(d1p)
+    (d1p, tl, ml, d1l), ms_prep = _timed(prep, reps=1)
+
+    def plain(nb):
+        outs = []
+        for b in range(nb):
+            c1 = ct1[b * B * 96:(b + 1) * B * 96]
+            fc, pc = api.pfe_precompute(c1, tl)
+            outs.append(api.pfe_open(shares[b * T * 144:(b + 1) * T * 144], d1p, ml, c1,
+                                     ct2[b * B * 576:(b + 1) * B * 576], fc, pc, wl.alpha))
+        return torch.cat(outs)
+
+    def fused():
+        fc, pc = api.pfe_precompute_batched(ct1, tl, B)
+        return api.pfe_open_prepared_batched(shares, T, d1l, ml, ct1, ct2, fc, pc, wl.alpha, B)
+    out_plain, ms_plain = _timed(lambda: plain(plain_batches), reps=1)
+    out_fused, ms_fused = _timed(fused, reps=1)
+    Wm.profile_enable(True)
+    fc, pc = api.pfe_precompute_batched(ct1, tl, B)
+    ph_pre = Wm.profile_read()
+    api.pfe_open_prepared_batched(shares, T, d1l, ml, ct1, ct2, fc, pc, wl.alpha, B)
+    ph_open = Wm.profile_read()
+    Wm.profile_enable(False)
+    ok = torch.equal(out_plain, msgs[:plain_batches * B * 576]) and torch.equal(out_fused, msgs)
+    plain_rate = plain_batches * B / (ms_plain * 1e-3)
+    fused_rate = batches * B / (ms_fused * 1e-3)
     return {"workload": "weighted_pfe decrypt", "B": B, "parties": n_parties, "total_weight": total_weight,
-            "committee": len(committee), "setup_s": round(t_setup, 1), "prepare_ms": round(ms_prep, 2),
-            "precompute_open_ms": round(ms, 2), "ctxts_per_s_cached_plain": round(B / (ms * 1e-3), 1),
-            "ctxts_per_s_cold_plain": round(B / ((ms + ms_prep) * 1e-3), 1),
-            "plaintexts_bit_exact": bool(torch.equal(out, msgs))}
+            "committee": T, "fused_batches": batches, "setup_s": round(t_setup, 1),
+            "prepare_ms": round(ms_prep, 2), "plain_ms_per_batch": round(ms_plain / plain_batches, 2),
+            "fused_ms_total": round(ms_fused, 2), "ctxts_per_s_cached_plain": round(plain_rate, 1),
+            "ctxts_per_s_cached_fused": round(fused_rate, 1), "fused_over_plain": round(fused_rate / plain_rate, 2),
+            "ctxts_per_s_cold_fused": round(batches * B / ((ms_fused + ms_prep) * 1e-3), 1),
+            "phases_ms_fused": {"precompute": {k: round(v, 2) for k, v in ph_pre.items()},
+                                "open": {k: round(v, 2) for k, v in ph_open.items()}},
+            "plaintexts_bit_exact": bool(ok)}
 
 
 def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batches=2, with_cpu=True):

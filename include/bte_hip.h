@@ -276,6 +276,18 @@ int bte_pfe_open(const bte_p1* shares, size_t n_accepted, const bte_p2_affine* d
                  const bte_fp6* mask_lines, const bte_p1_affine* ct1, const bte_fp12* ct2,
                  const bte_p1* first_cauchy, const bte_fp12* pairing_cauchy, const uint8_t* alpha,
                  size_t batch_size, bte_fp12* messages_out, void* stream);
+/* Fused forms over `count` batches of one committee (arrays count x B back to back, shares
+ * count x T): precompute_batch per batch, and open_batch against d1_lines = bte_g2_prepare_lines
+ * of d1_by_party (B x T tables, prepared once per committee and batch size). */
+int bte_pfe_precompute_batched(const bte_p1_affine* ct1, size_t batch_size, size_t count,
+                               const bte_fp6* total_lines, bte_p1* first_cauchy, bte_fp12* pairing_cauchy,
+                               void* stream);
+int bte_pfe_open_prepared_batched(const bte_p1* shares, size_t n_accepted, const bte_fp6* d1_lines,
+                                  const bte_fp6* mask_lines, const bte_p1_affine* ct1, const bte_fp12* ct2,
+                                  const bte_p1* first_cauchy, const bte_fp12* pairing_cauchy,
+                                  const uint8_t* alpha, size_t batch_size, size_t count,
+                                  bte_fp12* messages_out, void* stream);
+
 /* decryption.rs:230-251 partial_decrypt for N parties: sigma_j = sum_i g_{j,i} ct1_i with the
  * N x B fractions g (canonical scalars, device). */
 int bte_pfe_partial_decrypt(const bte_p1_affine* ct1, size_t batch_size, const uint8_t* fractions,

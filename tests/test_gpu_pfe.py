@@ -49,3 +49,35 @@ def test_pfe_decrypts_and_cauchy_matches_literal(api, B):
     for i in range(B):
         e = sum(wl.r[k] * kappa[k] * pow(wl.alpha[k] - wl.alpha[i], -1, O.R) for k in range(B) if k != i) % O.R
         assert got[i] == O.f12_pow(g, e), "pairing cauchy slot %d" % i
+
+
+def test_pfe_fused_batches_equal_alone(api):
+    """Three batches through the fused precompute / prepared opening give the plain per-batch
+    results and the plaintexts."""
+    import torch
+
+    from weighted_bte_amd import workloads
+
+    weights = [2, 1, 3, 2]
+    B, K = 4, 3
+    wl = workloads.PfeWorkload(weights, threshold=4, batch_size=B, seed=21)
+    committee = workloads.heaviest_first(weights, wl.t)
+    T = len(committee)
+    ct1, ct2, msgs = wl.encrypt(K)
+    d1p, tl, ml = api.pfe_prepare(wl.material, wl.W, B, wl.offsets, committee, wl.alpha, wl.global_key)
+    shares = []
+    for b in range(K):
+        sig = api.pfe_partial_decrypt(ct1[b * B * 96:(b + 1) * B * 96], wl.g)
+        shares.append(torch.cat([sig[144 * j:144 * (j + 1)] for j in committee]))
+    shares = torch.cat(shares)
+    fc, pc = api.pfe_precompute_batched(ct1, tl, B)
+    fused = api.pfe_open_prepared_batched(shares, T, api.prepare_g2_lines(d1p), ml, ct1, ct2, fc, pc, wl.alpha, B)
+    assert api.to_host(fused) == api.to_host(msgs)
+    for b in range(K):
+        c1 = ct1[b * B * 96:(b + 1) * B * 96]
+        f1, p1 = api.pfe_precompute(c1, tl)
+        assert api.to_host(f1) == api.to_host(fc[b * B * 144:(b + 1) * B * 144])
+        assert api.to_host(p1) == api.to_host(pc[b * B * 576:(b + 1) * B * 576])
+        alone = api.pfe_open(shares[b * T * 144:(b + 1) * T * 144], d1p, ml, c1, ct2[b * B * 576:(b + 1) * B * 576],
+                             f1, p1, wl.alpha)
+        assert api.to_host(alone) == api.to_host(fused[b * B * 576:(b + 1) * B * 576])

@@ -74,6 +74,8 @@ _SIGNATURES = {
     "bte_miller_loop_prepared_segmented": [_vp, _vp, _sz, _vp, _sz, _vp, _vp],
     "bte_btx_decrypt_requests": [_vp, _sz, _vp],
     "bte_btx_open_prepared_batched": [_vp, _sz, _vp, _vp, _vp, _vp, _sz, _sz, _vp, _vp],
+    "bte_pfe_precompute_batched": [_vp, _sz, _sz, _vp, _vp, _vp, _vp],
+    "bte_pfe_open_prepared_batched": [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _sz, _vp, _vp],
     "bte_multi_miller_loop_prepared": [_vp, _vp, _sz, _vp, _sz, _vp, _vp],
     "bte_profile_enable": [_i],
     "bte_profile_read": [ctypes.c_char_p, _sz],

@@ -509,6 +509,30 @@ def pfe_prepare(material, total_weight, batch_size, party_offsets, accepted, alp
     return d1p, tl, ml
 
 
+def pfe_precompute_batched(ct1_affine, total_lines, batch_size):
+    """precompute_batch for count = len / B batches of one committee in one fused pass."""
+    ct1 = _dev(ct1_affine)
+    n = ct1.numel() // G1_AFFINE
+    count = n // batch_size
+    fc = _empty(n * G1_JAC, ct1)
+    pc = _empty(n * FP12, ct1)
+    call("bte_pfe_precompute_batched", _ptr(ct1), batch_size, count, _ptr(_dev(total_lines)), _ptr(fc), _ptr(pc),
+         _stream())
+    return fc, pc
+
+
+def pfe_open_prepared_batched(shares_jac, n_accepted, d1_lines, mask_lines, ct1_affine, ct2, first_cauchy,
+                              pairing_cauchy, alpha, batch_size):
+    """open_batch for count batches against d1_lines = prepare_g2_lines(d1_by_party)."""
+    ct2 = _dev(ct2)
+    n = ct2.numel() // FP12
+    out = _empty(n * FP12, ct2)
+    call("bte_pfe_open_prepared_batched", _ptr(_dev(shares_jac)), n_accepted, _ptr(_dev(d1_lines)),
+         _ptr(_dev(mask_lines)), _ptr(_dev(ct1_affine)), _ptr(ct2), _ptr(_dev(first_cauchy)),
+         _ptr(_dev(pairing_cauchy)), _host_scalars(alpha), batch_size, n // batch_size, _ptr(out), _stream())
+    return out
+
+
 def pfe_precompute(ct1_affine, total_lines):
     """precompute_batch (decryption.rs:540-573) -> (first_cauchy, pairing_cauchy)."""
     ct1 = _dev(ct1_affine)

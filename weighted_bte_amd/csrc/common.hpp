@@ -60,6 +60,11 @@ int miller_prepared_segmented(const void* const* tables, const uint64_t* seg_off
 int multi_miller_prepared(const void* const* pair_tables, const void* p, size_t npairs, const uint64_t* seg_offsets,
                           size_t nseg, void* out, hipStream_t s);
 
+// Cyclotomic powers over K batches of B values with one exponent per slot i (element b B + i):
+// canonical scalars (HOST, B x 32 bytes) or small signed integers (device, B) (fft.hip).
+int cyc_pow_scalars_tiled(fp12* v, const uint8_t* host_scalars, size_t B, size_t K, hipStream_t s);
+int cyc_pow_small_tiled(fp12* v, const int64_t* coefficients, size_t B, size_t K, hipStream_t s);
+
 // Segment holding flat index g: off[f] <= g < off[f + 1], off[0] = 0, count segments.
 __device__ inline uint32_t seg_of(const uint64_t* __restrict__ off, uint32_t count, uint64_t g) {
   uint32_t lo = 0, hi = count;

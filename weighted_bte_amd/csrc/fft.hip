@@ -727,6 +727,44 @@ extern "C" int bte_cyclotomic_pow_scalars(bte_fp12* values, const uint8_t* host_
   return BTE_OK;
 }
 
+// K batches of B values with one exponent per slot (element b B + i uses exponent i): lanes run
+// slot-major (consecutive lanes = one slot of consecutive batches) so a wavefront shares its
+// exponent whenever K >= 64.
+__global__ void __launch_bounds__(64) k_cyc_pow_plans_tiled(fp12* __restrict__ v, size_t B, size_t K,
+                                                            const cyc_plan* __restrict__ plans) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= B * K) return;
+  const size_t i = g / K, b = g - i * K;
+  v[b * B + i] = cyclo_pow_plan(v[b * B + i], plans + i);
+}
+
+__global__ void __launch_bounds__(64) k_cyc_pow_small_tiled(fp12* __restrict__ v, size_t B, size_t K,
+                                                            const int64_t* __restrict__ c) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= B * K) return;
+  const size_t i = g / K, b = g - i * K;
+  v[b * B + i] = cyclo_pow_small(v[b * B + i], c[i]);
+}
+
+int bte::cyc_pow_scalars_tiled(fp12* v, const uint8_t* host_scalars, size_t B, size_t K, hipStream_t s) {
+  if (B == 0 || K == 0) return BTE_OK;
+  cyc_plan* plans;
+  int rc = upload_plans(host_scalars, B, &plans, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_cyc_pow_plans_tiled, dim3(blocks_for(B * K, 64)), dim3(64), 0, s, v, B, K,
+                     (const cyc_plan*)plans);
+  (void)hipFreeAsync(plans, s);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+int bte::cyc_pow_small_tiled(fp12* v, const int64_t* coefficients, size_t B, size_t K, hipStream_t s) {
+  if (B == 0 || K == 0) return BTE_OK;
+  hipLaunchKernelGGL(k_cyc_pow_small_tiled, dim3(blocks_for(B * K, 64)), dim3(64), 0, s, v, B, K, coefficients);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
 extern "C" int bte_cyclotomic_pow_small(bte_fp12* values, const int64_t* coefficients, size_t n,
                                         void* stream) {
   if (n == 0) return BTE_OK;

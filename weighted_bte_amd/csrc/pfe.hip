@@ -28,6 +28,10 @@ int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine* q, size_t
 int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
 int bte_final_exp_hard(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
 int bte_lagrange_at_zero(size_t total_weight, const uint64_t* selected, size_t n, uint8_t* out);
+int bte_fft_g1_batched(bte_p1* values, size_t n, size_t count, int inverse, int scaled, void* stream);
+int bte_fft_cyclotomic_batched(bte_fp12* values, size_t n, size_t count, int inverse, void* stream);
+int bte_miller_loop_prepared_tiled(const bte_fp6* lines, size_t ntables, const bte_p1_affine* p, size_t n,
+                                   bte_fp12* out, void* stream);
 }
 
 namespace bte {
@@ -71,6 +75,41 @@ __global__ void k_g1_shift_neg(const g1j* __restrict__ in, size_t n, g1j* __rest
 __global__ void k_cyc_shift_conj(const fp12* __restrict__ in, size_t n, fp12* __restrict__ out) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i < n) out[i] = fp12_conj(in[(i + n - 1) % n]);
+}
+
+// Batched forms: element i of batch b sits at b B + i; slot-major lanes share the coefficient.
+__global__ void __launch_bounds__(64) k_g1_mul_small_tiled(g1j* __restrict__ v, const int64_t* __restrict__ c, size_t B,
+                                                           size_t K) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= B * K) return;
+  const size_t i = g / K, b = g - i * K;
+  v[b * B + i] = mul_signed_small(v[b * B + i], c[i]);
+}
+
+__global__ void k_g1_shift_neg_tiled(const g1j* __restrict__ in, size_t B, size_t n, g1j* __restrict__ out) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const size_t b = g / B, i = g % B;
+  out[g] = jac_neg(in[b * B + (i + B - 1) % B]);
+}
+
+__global__ void k_cyc_shift_conj_tiled(const fp12* __restrict__ in, size_t B, size_t n, fp12* __restrict__ out) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const size_t b = g / B, i = g % B;
+  out[g] = fp12_conj(in[b * B + (i + B - 1) % B]);
+}
+
+// left[(b B + slot) T + p] = shares2B[b T + p] - cauchy[b B + slot]; tab = line table of the
+// committee's d1 point (slot, p).
+__global__ void k_pfe_left_tiled(const g1j* __restrict__ shares2b, const g1j* __restrict__ cauchy,
+                                 const bte_fp6* __restrict__ d1_lines, size_t T, size_t B, size_t count,
+                                 g1j* __restrict__ out, const bte_fp6** __restrict__ tab) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g >= T * B * count) return;
+  const size_t p = g % T, slot = (g / T) % B, b = g / (T * B);
+  out[g] = jac_sub(shares2b[b * T + p], cauchy[b * B + slot]);
+  tab[g] = d1_lines + (slot * T + p) * 68;
 }
 
 template <class F> __global__ void k_jac_from_aff(const aff<F>* __restrict__ in, size_t n, jac<F>* __restrict__ out) {
@@ -342,6 +381,119 @@ extern "C" int bte_pfe_open(const bte_p1* shares, size_t n_accepted, const bte_p
   hipLaunchKernelGGL(k_pfe_unmask, dim3(blocks_for(B, 64)), dim3(64), 0, s, (const fp12*)ct2, (const fp12*)mask, B,
                      (fp12*)messages_out);
   BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+// precompute_batch for `count` batches of B ciphertexts of one committee, every stage one launch
+// over all batches (same outputs as bte_pfe_precompute per batch).
+extern "C" int bte_pfe_precompute_batched(const bte_p1_affine* ct1, size_t batch_size, size_t count,
+                                          const bte_fp6* total_lines, bte_p1* first_cauchy, bte_fp12* pairing_cauchy,
+                                          void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t B = batch_size, K = count, n = B * K;
+  if (!ct1 || !total_lines || !first_cauchy || !pairing_cauchy) return set_error(BTE_EINVAL, "pfe_precompute: null");
+  if (B == 0 || (B & (B - 1))) return set_error(BTE_EINVAL, "pfe_precompute: InvalidBatchSize(%zu)", B);
+  if (K == 0) return BTE_OK;
+  std::vector<int64_t> coef(B);
+  for (size_t k = 0; k < B; ++k) coef[k] = (int64_t)B - 1 - 2 * (int64_t)k;  // decryption.rs:198-200
+  PScratch sc(s);
+  int64_t* d_coef;
+  g1j* spec;
+  fp12 *ml, *cyc;
+  PFE_TRY(sc.upload(&d_coef, coef.data(), B));
+  PFE_TRY(sc.done());
+  PFE_TRY(sc.get(&spec, n));
+  PFE_TRY(sc.get(&ml, n));
+  PFE_TRY(sc.get(&cyc, n));
+  PhaseTimer pt(s);
+  pt.begin();
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_jac_from_aff<fp>), dim3(blocks_for(n, 256)), dim3(256), 0, s, (const g1a*)ct1, n, spec);
+  PFE_TRY(bte_fft_g1_batched((bte_p1*)spec, B, K, 0, 0, s));
+  hipLaunchKernelGGL(k_g1_mul_small_tiled, dim3(blocks_for(n, 64)), dim3(64), 0, s, spec, (const int64_t*)d_coef, B, K);
+  hipLaunchKernelGGL(k_g1_shift_neg_tiled, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const g1j*)spec, B, n,
+                     (g1j*)first_cauchy);
+  PFE_TRY(bte_fft_g1_batched(first_cauchy, B, K, 1, 0, s));
+  pt.mark("group_cauchy");
+  PFE_TRY(bte_miller_loop_prepared_tiled(total_lines, B, ct1, n, (bte_fp12*)ml, s));
+  PFE_TRY(bte_final_exp_easy((const bte_fp12*)ml, n, (bte_fp12*)cyc, s));
+  pt.mark("miller_easy");
+  PFE_TRY(bte_fft_cyclotomic_batched((bte_fp12*)cyc, B, K, 0, s));
+  PFE_TRY(cyc_pow_small_tiled(cyc, d_coef, B, K, s));
+  hipLaunchKernelGGL(k_cyc_shift_conj_tiled, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const fp12*)cyc, B, n,
+                     (fp12*)pairing_cauchy);
+  PFE_TRY(bte_fft_cyclotomic_batched(pairing_cauchy, B, K, 1, s));
+  pt.mark("cyclotomic_cauchy");
+  BTE_CHECK_LAUNCH();
+  if (pt.on) {
+    BTE_HIP(hipStreamSynchronize(s));
+    pt.publish();
+  }
+  return BTE_OK;
+}
+
+// open_batch for `count` batches of one committee against the line tables of d1_by_party
+// (bte_g2_prepare_lines over its B x T points, committee state): shares count x T, ct1 / ct2 /
+// first_cauchy / pairing_cauchy count x B; alpha B canonical scalars (host).
+extern "C" int bte_pfe_open_prepared_batched(const bte_p1* shares, size_t n_accepted, const bte_fp6* d1_lines,
+                                             const bte_fp6* mask_lines, const bte_p1_affine* ct1,
+                                             const bte_fp12* ct2, const bte_p1* first_cauchy,
+                                             const bte_fp12* pairing_cauchy, const uint8_t* alpha, size_t batch_size,
+                                             size_t count, bte_fp12* messages_out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t B = batch_size, T = n_accepted, K = count, n = B * K;
+  if (!shares || !d1_lines || !mask_lines || !ct1 || !ct2 || !first_cauchy || !pairing_cauchy || !alpha ||
+      !messages_out)
+    return set_error(BTE_EINVAL, "pfe_open: null pointer");
+  if (B == 0 || T == 0) return set_error(BTE_EINVAL, "pfe_open: empty batch or committee");
+  if (K == 0) return BTE_OK;
+  std::vector<uint64_t> off(n + 1);
+  for (size_t i = 0; i <= n; ++i) off[i] = i * T;
+  std::vector<uint8_t> aa1(32 * B);
+  for (size_t i = 0; i < B; ++i) {
+    bteh::Fr a;
+    memcpy(a.l, alpha + 32 * i, 32);
+    bteh::Fr p = bteh::fr_mul(a, bteh::fr_add(a, bteh::fr_from_u64(1)));
+    memcpy(&aa1[32 * i], p.l, 32);
+  }
+  PScratch sc(s);
+  g1j *sh2, *left;
+  g1a* left_aff;
+  const bte_fp6** tab;
+  fp12 *ml, *easy, *prod, *comb, *mask;
+  PFE_TRY(sc.get(&sh2, K * T));
+  PFE_TRY(sc.get(&left, n * T));
+  PFE_TRY(sc.get(&left_aff, n * T));
+  PFE_TRY(sc.get(&tab, n * T));
+  PFE_TRY(sc.get(&ml, 2 * n));
+  PFE_TRY(sc.get(&easy, 2 * n));
+  PFE_TRY(sc.get(&prod, n));
+  PFE_TRY(sc.get(&comb, n));
+  PFE_TRY(sc.get(&mask, n));
+  PhaseTimer pt(s);
+  pt.begin();
+  BTE_HIP(hipMemcpyAsync(sh2, shares, K * T * sizeof(g1j), hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_g1_mul_small_const, dim3(blocks_for(K * T, 64)), dim3(64), 0, s, sh2, (int64_t)(2 * B), K * T);
+  hipLaunchKernelGGL(k_pfe_left_tiled, dim3(blocks_for(n * T, 256)), dim3(256), 0, s, (const g1j*)sh2,
+                     (const g1j*)first_cauchy, d1_lines, T, B, K, left, tab);
+  PFE_TRY(bte_g1_batch_normalize((const bte_p1*)left, n * T, (bte_p1_affine*)left_aff, s));
+  PFE_TRY(multi_miller_prepared((const void* const*)tab, left_aff, n * T, off.data(), n, ml, s));
+  PFE_TRY(bte_miller_loop_prepared_tiled(mask_lines, B, ct1, n, (bte_fp12*)(ml + n), s));
+  pt.mark("miller");
+  PFE_TRY(bte_final_exp_easy((const bte_fp12*)ml, 2 * n, (bte_fp12*)easy, s));
+  hipLaunchKernelGGL(k_fp12_mul_pairs, dim3(blocks_for(n, 64)), dim3(64), 0, s, (const fp12*)easy,
+                     (const fp12*)pairing_cauchy, n, prod);
+  PFE_TRY(cyc_pow_scalars_tiled(prod, aa1.data(), B, K, s));
+  hipLaunchKernelGGL(k_pfe_combine, dim3(blocks_for(n, 64)), dim3(64), 0, s, (const fp12*)(easy + n),
+                     (const fp12*)prod, n, comb);
+  PFE_TRY(bte_final_exp_hard((const bte_fp12*)comb, n, (bte_fp12*)mask, s));
+  hipLaunchKernelGGL(k_pfe_unmask, dim3(blocks_for(n, 64)), dim3(64), 0, s, (const fp12*)ct2, (const fp12*)mask, n,
+                     (fp12*)messages_out);
+  pt.mark("final");
+  BTE_CHECK_LAUNCH();
+  if (pt.on) {
+    BTE_HIP(hipStreamSynchronize(s));
+    pt.publish();
+  }
   return BTE_OK;
 }
 

@@ -170,10 +170,10 @@ class PfeWorkload:
         self.stream = st
         self.gt = gt_generator_bytes()
 
-    def encrypt(self):
-        """B messages gT^s with ct1 = r G1 and ct2 = gT^(s + z beta_0 r) (encryption key
-        gT^(z beta_0))."""
-        B = self.B
+    def encrypt(self, count=1):
+        """count batches of B messages gT^s with ct1 = r G1 and ct2 = gT^(s + z beta_0 r)
+        (encryption key gT^(z beta_0))."""
+        B = self.B * count
         self.s = self.stream.scalars(B)
         self.r = self.stream.scalars(B)
         ct1 = api.g1_generator_mul(api.to_device(inputs.pack_scalars(self.r)))
