@@ -286,7 +286,8 @@ def bench_pfe(B=512, n_parties=128, total_weight=4096, batches=128, plain_batche
             "committee": T, "fused_batches": batches, "setup_s": round(t_setup, 1),
             "prepare_ms": round(ms_prep, 2), "plain_ms_per_batch": round(ms_plain / plain_batches, 2),
             "fused_ms_total": round(ms_fused, 2), "ctxts_per_s_cached_plain": round(plain_rate, 1),
-            "ctxts_per_s_cached_fused": round(fused_rate, 1), "fused_over_plain": round(fused_rate / plain_rate, 2),
+            "ctxts_per_s_cached_fused": round(fused_rate, 1),
+        "ctxts_per_s_cached_graph": round(n / t_graph, 1), "fused_over_plain": round(fused_rate / plain_rate, 2),
             "ctxts_per_s_cold_fused": round(batches * B / ((ms_fused + ms_prep) * 1e-3), 1),
             "phases_ms_fused": {"precompute": {k: round(v, 2) for k, v in ph_pre.items()},
                                 "open": {k: round(v, 2) for k, v in ph_open.items()}},
@@ -355,7 +356,21 @@ def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batche
     out_plain, t_plain = timed(lambda: plain(plain_batches))
     fused()
     out_fused, t_fused = timed(fused)
-    ok = bool(torch.equal(out_plain, msgs[:plain_batches * B * 576]) and torch.equal(out_fused, msgs))
+    # the same fused step as a launch-only plan captured in a HIP graph (side stream)
+    gstream = torch.cuda.Stream()
+    gstream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(gstream):
+        plan = api.BtxPlan(B, batches, T, lines, key_lines, valid)
+        out_graph = torch.empty(batches * B * 576, dtype=torch.uint8, device="cuda")
+        plan.launch_graph(ct1, shares, ct2, out_graph)
+        gstream.synchronize()
+        t = time.perf_counter()
+        plan.launch_graph(ct1, shares, ct2, out_graph)
+        gstream.synchronize()
+        t_graph = time.perf_counter() - t
+    graph_ok = torch.equal(out_graph, msgs)
+    plan.close()
+    ok = bool(torch.equal(out_plain, msgs[:plain_batches * B * 576]) and torch.equal(out_fused, msgs) and graph_ok)
     # phase breakdown of one fused pass (HIP events on the library's stream)
     W.profile_enable(True)
     beta = api.btx_precompute_batched(ct1, valid, lines, B)
@@ -387,6 +402,7 @@ def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batche
         "fused_ms_total": round(t_fused * 1e3, 2),
         "ctxts_per_s_cached_plain": round(plain_batches * B / t_plain, 1),
         "ctxts_per_s_cached_fused": round(fused_rate, 1),
+        "ctxts_per_s_cached_graph": round(n / t_graph, 1),
         "ctxts_per_s_cold_plain": round(B / (t_prepare + t_plain / plain_batches), 1),
         "ctxts_per_s_cold_fused": round(n / (t_prepare + t_fused), 1),
         "fused_over_plain": round(fused_rate / (plain_batches * B / t_plain), 2),

@@ -248,6 +248,22 @@ int bte_btx_open_prepared_batched(const bte_p1* shares, size_t n_accepted, const
                                   const bte_fp12* ct2, const bte_fp12* beta, const uint8_t* valid,
                                   size_t batch_size, size_t count, bte_fp12* messages_out, void* stream);
 
+/* Launch-only plan of the fused decrypt step (bte_btx_precompute_batched followed by
+ * bte_btx_open_prepared_batched for `count` batches of one committee, with the same outputs).
+ * Creation allocates every buffer, uploads the validity flags (count x B, HOST) and builds the
+ * domain tables; bte_btx_plan_run then only enqueues kernels.  bte_btx_plan_launch_graph
+ * captures one run into a HIP graph on its first call (stream must not be the null stream; the
+ * graph stays bound to the buffers of that call, later calls must pass the same pointers) and
+ * launches the graph.  No counterpart in the reference (its steps are rayon loops). */
+typedef struct bte_btx_plan bte_btx_plan;
+int bte_btx_plan_create(size_t batch_size, size_t count, size_t n_accepted, const bte_fp6* kernel_lines,
+                        const bte_fp6* key_lines, const uint8_t* valid, bte_btx_plan** plan, void* stream);
+int bte_btx_plan_run(bte_btx_plan* plan, const bte_p1_affine* ct1, const bte_p1* shares, const bte_fp12* ct2,
+                     bte_fp12* messages_out, void* stream);
+int bte_btx_plan_launch_graph(bte_btx_plan* plan, const bte_p1_affine* ct1, const bte_p1* shares,
+                              const bte_fp12* ct2, bte_fp12* messages_out, void* stream);
+int bte_btx_plan_destroy(bte_btx_plan* plan);
+
 /* decryption.rs:205-233 partial_decrypt for N parties at once: sigma_j = sum_i q_j^i ct1_i,
  * i = 1..B.  q = N canonical scalars (HOST); sigma_out = N G1 Jacobian (device). */
 int bte_btx_partial_decrypt(const bte_p1_affine* ct1, size_t batch_size, const uint8_t* q,

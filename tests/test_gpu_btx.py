@@ -183,3 +183,48 @@ def test_multi_miller_prepared_uneven_segments(api):
     call("bte_multi_miller_loop_prepared", ctypes.c_void_p(tabs.data_ptr()), ctypes.c_void_p(P.data_ptr()), n,
          hoff, len(arities), ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
     assert api.to_host(out) == api.to_host(want)
+
+
+@pytest.mark.parametrize("B,K", [(8, 3), (64, 8)])
+def test_btx_plan_run_and_graph_replays(api, B, K):
+    """The launch-only plan gives the fused entry points' messages; its HIP graph, replayed three
+    times with eager work and synchronisation between replays, gives them every time."""
+    import torch
+
+    from weighted_bte_amd import workloads
+
+    weights = [2, 3, 1, 4]
+    wl = workloads.BtxWorkload(weights, threshold=5, max_batch=B, seed=31 + B)
+    committee = workloads.heaviest_first(weights, wl.t)
+    T = len(committee)
+    ct1, ct2, msgs = wl.encrypt(B, K)
+    valid = [True] * (B * K)
+    valid[1] = False
+    raw = bytearray(api.to_host(ct1))
+    raw[96:192] = bytes(96)
+    ct1 = api.to_device(bytes(raw))
+    lines, keys = api.btx_prepare(wl.material, wl.W, wl.L, wl.offsets, committee, B)
+    key_lines = api.prepare_g2_lines(keys)
+    sh = []
+    for b in range(K):
+        sig = api.btx_partial_decrypt(ct1[b * B * 96:(b + 1) * B * 96], wl.q)
+        sh.append(torch.cat([sig[144 * j:144 * (j + 1)] for j in committee]))
+    shares = torch.cat(sh)
+    beta = api.btx_precompute_batched(ct1, valid, lines, B)
+    want = api.to_host(api.btx_open_prepared_batched(shares, T, key_lines, ct2, beta, valid, B))
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        plan = api.BtxPlan(B, K, T, lines, key_lines, valid)
+        assert api.to_host(plan.run(ct1, shares, ct2)) == want
+        out = torch.empty(B * K * 576, dtype=torch.uint8, device="cuda")
+        for rep in range(3):
+            out.zero_()
+            plan.launch_graph(ct1, shares, ct2, out)
+            stream.synchronize()
+            assert api.to_host(out) == want, "replay %d" % rep
+            api.g1_generator_mul(api.to_device((rep + 5).to_bytes(32, "little")))  # eager work between
+            torch.cuda.synchronize()
+        plan.close()
+    one = O.f12_to_bytes(O.F12_ONE)
+    assert want[576:1152] == one

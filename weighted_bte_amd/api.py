@@ -477,6 +477,45 @@ def btx_open_prepared_batched(shares_jac, n_accepted, key_lines, ct2, beta, vali
     return out
 
 
+class BtxPlan:
+    """Launch-only plan of the fused decrypt step for `count` batches of B ciphertexts of one
+    committee (kernel_lines / key_lines from btx_prepare and prepare_g2_lines(keys)); run()
+    enqueues kernels only, launch_graph() replays a HIP graph of one run (bound to the buffers of
+    its first call; the current stream must not be the null stream)."""
+
+    def __init__(self, batch_size, count, n_accepted, kernel_lines, key_lines, valid):
+        self._keep = (_dev(kernel_lines), _dev(key_lines))
+        self.B, self.K, self.T = batch_size, count, n_accepted
+        v = (ctypes.c_uint8 * (batch_size * count))(*[1 if x else 0 for x in valid])
+        h = ctypes.c_void_p(0)
+        call("bte_btx_plan_create", batch_size, count, n_accepted, _ptr(self._keep[0]), _ptr(self._keep[1]), v,
+             ctypes.byref(h), _stream())
+        self.handle = h
+
+    def _args(self, ct1, shares, ct2, out):
+        return (self.handle, _ptr(_dev(ct1)), _ptr(_dev(shares)), _ptr(_dev(ct2)), _ptr(out), _stream())
+
+    def run(self, ct1, shares, ct2, out=None):
+        out = out if out is not None else _empty(self.B * self.K * FP12, ct2)
+        call("bte_btx_plan_run", *self._args(ct1, shares, ct2, out))
+        return out
+
+    def launch_graph(self, ct1, shares, ct2, out):
+        call("bte_btx_plan_launch_graph", *self._args(ct1, shares, ct2, out))
+        return out
+
+    def close(self):
+        if self.handle:
+            call("bte_btx_plan_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def btx_partial_decrypt(ct1_affine, q_scalars):
     """partial_decrypt (decryption.rs:205-233) for every party key q_j -> sigma (N x G1 Jacobian)."""
     ct1 = _dev(ct1_affine)

@@ -164,15 +164,6 @@ __global__ void k_open_pairs(const g1a* __restrict__ shares, const g2a* __restri
   Q[g] = keys[i * T + p];
 }
 
-__global__ void k_gather_pairs(const g1a* __restrict__ P, const bte_fp6* const* __restrict__ tab,
-                               const uint32_t* __restrict__ src, uint64_t n, g1a* __restrict__ P2,
-                               const bte_fp6** __restrict__ tab2) {
-  const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  P2[g] = P[src[g]];
-  tab2[g] = tab[src[g]];
-}
-
 template <class T> __global__ void k_tile(const T* __restrict__ in, size_t n, size_t reps, T* __restrict__ out) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i < n * reps) out[i] = in[i % n];
@@ -693,40 +684,18 @@ extern "C" int bte_btx_open_prepared_batched(const bte_p1* shares, size_t n_acce
   BTE_TRY(sc.get(&alpha, K * B));
   BTE_TRY(sc.upload(&d_valid, valid, K * B));
   BTE_TRY(sc.uploads_done());
-  // slot j owns pairs [j T, j T + T) when valid, none otherwise (its message is one)
-  std::vector<uint64_t> off(K * B + 1);
-  off[0] = 0;
-  for (size_t j = 0; j < K * B; ++j) off[j + 1] = off[j] + (valid[j] ? T : 0);
   PhaseTimer pt(s);
   pt.begin();
   BTE_TRY(bte_g1_batch_normalize(shares, K * T, (bte_p1_affine*)sa, s));
   hipLaunchKernelGGL(k_open_pairs_prepared, dim3(blocks_for(npairs, 256)), dim3(256), 0, s, (const g1a*)sa,
                      key_lines, T, B, K, P, tab);
-  std::vector<uint64_t> seg(K * B + 1);
-  bool all_valid = true;
-  for (size_t j = 0; j < K * B; ++j) all_valid = all_valid && valid[j];
-  if (all_valid) {
-    for (size_t j = 0; j <= K * B; ++j) seg[j] = j * T;
-    BTE_TRY(multi_miller_prepared((const void* const*)tab, P, npairs, seg.data(), K * B, ml, s));
-  } else {
-    // the pairs of valid slots, gathered densely (segment j = off[j] .. off[j+1])
-    std::vector<uint32_t> src;
-    src.reserve(off[K * B]);
-    for (size_t j = 0; j < K * B; ++j)
-      if (valid[j])
-        for (size_t p = 0; p < T; ++p) src.push_back((uint32_t)(j * T + p));
-    g1a* P2;
-    const bte_fp6** tab2;
-    uint32_t* d_src;
-    BTE_TRY(sc.get(&P2, std::max<size_t>(src.size(), 1)));
-    BTE_TRY(sc.get(&tab2, std::max<size_t>(src.size(), 1)));
-    BTE_TRY(sc.upload(&d_src, src.data(), std::max<size_t>(src.size(), 1)));
-    BTE_TRY(sc.uploads_done());
-    if (!src.empty())
-      hipLaunchKernelGGL(k_gather_pairs, dim3(blocks_for(src.size(), 256)), dim3(256), 0, s, (const g1a*)P,
-                         (const bte_fp6* const*)tab, (const uint32_t*)d_src, (uint64_t)src.size(), P2, tab2);
-    BTE_TRY(multi_miller_prepared((const void* const*)tab2, P2, src.size(), off.data(), K * B, ml, s));
+  // slot j owns pairs [j T, j T + T) when valid; an invalid slot's segment is empty
+  std::vector<uint64_t> lo(K * B), hi(K * B);
+  for (size_t j = 0; j < K * B; ++j) {
+    lo[j] = j * T;
+    hi[j] = valid[j] ? j * T + T : j * T;
   }
+  BTE_TRY(multi_miller_prepared_ranges((const void* const*)tab, P, lo.data(), hi.data(), K * B, ml, s));
   pt.mark("multi_miller");
   BTE_TRY(bte_final_exp((const bte_fp12*)ml, K * B, (bte_fp12*)alpha, s));
   pt.mark("final_exp");
@@ -773,5 +742,157 @@ extern "C" int bte_btx_partial_decrypt(const bte_p1_affine* ct1, size_t batch_si
   BTE_TRY(sc.upload(&d_off, off.data(), N + 1));
   BTE_TRY(sc.uploads_done());
   BTE_TRY(g1_msm_batched_gather(ct1, B, d_pidx, d_pw, nullptr, N * B, d_off, N, 0, sigma_out, s));
+  return BTE_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launch-only plan of the fused decrypt step (precompute_batch + open_batch over `count`
+// batches of one committee, the opening keys prepared), and its HIP graph.  Creation allocates
+// every buffer, uploads the slot validity and the multi-Miller chunk ranges and builds the FFT
+// domain tables; a run then only enqueues kernels, so it can be captured and replayed.
+// ---------------------------------------------------------------------------------------------
+struct bte_btx_plan {
+  size_t B = 0, K = 0, T = 0, m = 0, nchunks = 0;
+  const bte_fp6* lines = nullptr;
+  const bte_fp6* key_lines = nullptr;
+  g1j* c = nullptr;
+  g1a *ca = nullptr, *sa = nullptr, *P = nullptr;
+  fp12 *ml = nullptr, *conv = nullptr, *pref = nullptr, *beta = nullptr, *mm = nullptr, *alpha = nullptr,
+       *part = nullptr;
+  const bte_fp6** tab = nullptr;
+  uint8_t* valid = nullptr;
+  uint32_t* cyctab = nullptr;
+  uint64_t *lo = nullptr, *hi = nullptr, *seg = nullptr;
+  std::vector<void*> allocs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* bound[4] = {nullptr, nullptr, nullptr, nullptr};
+  template <class T> int alloc(T** p, size_t count) {
+    hipError_t e = hipMalloc((void**)p, (count ? count : 1) * sizeof(T));
+    if (e != hipSuccess) return set_error(BTE_ENOMEM, "btx plan: %s", hipGetErrorString(e));
+    allocs.push_back((void*)*p);
+    return BTE_OK;
+  }
+  ~bte_btx_plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void* p : allocs) (void)hipFree(p);
+  }
+};
+
+extern "C" int bte_btx_plan_create(size_t batch_size, size_t count, size_t n_accepted, const bte_fp6* kernel_lines,
+                                   const bte_fp6* key_lines, const uint8_t* valid, bte_btx_plan** out,
+                                   void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!out || !kernel_lines || !key_lines || !valid) return set_error(BTE_EINVAL, "btx_plan_create: null pointer");
+  if (batch_size == 0 || count == 0 || n_accepted == 0) return set_error(BTE_EINVAL, "btx_plan_create: empty shape");
+  *out = nullptr;
+  bte_btx_plan* p = new bte_btx_plan();
+  p->B = batch_size;
+  p->K = count;
+  p->T = n_accepted;
+  p->m = next_pow2(2 * batch_size);
+  p->lines = kernel_lines;
+  p->key_lines = key_lines;
+  const size_t M = p->m * p->K, S = p->B * p->K, NP = S * p->T;
+  std::vector<uint64_t> lo(S), hi(S);
+  for (size_t j = 0; j < S; ++j) {
+    lo[j] = j * p->T;
+    hi[j] = valid[j] ? j * p->T + p->T : j * p->T;
+  }
+  MMChunks ch;
+  mm_chunks(lo.data(), hi.data(), S, ch);
+  p->nchunks = ch.lo.size();
+  const size_t ctab = cyc_tab_bytes(p->m, p->K);
+  int rc = BTE_OK;
+  do {
+    if ((rc = p->alloc(&p->c, M)) || (rc = p->alloc(&p->ca, M)) || (rc = p->alloc(&p->ml, M)) ||
+        (rc = p->alloc(&p->conv, M)) || (rc = p->alloc(&p->pref, S)) || (rc = p->alloc(&p->beta, S)) ||
+        (rc = p->alloc(&p->sa, p->K * p->T)) || (rc = p->alloc(&p->P, NP)) || (rc = p->alloc(&p->tab, NP)) ||
+        (rc = p->alloc(&p->mm, S)) || (rc = p->alloc(&p->alpha, S)) || (rc = p->alloc(&p->valid, S)) ||
+        (rc = p->alloc(&p->lo, p->nchunks)) || (rc = p->alloc(&p->hi, p->nchunks)) ||
+        (rc = p->alloc(&p->seg, ch.seg.size())) || (rc = p->alloc(&p->part, p->nchunks)))
+      break;
+    if (ctab && (rc = p->alloc((uint8_t**)&p->cyctab, ctab))) break;
+    if (hipMemcpy(p->valid, valid, S, hipMemcpyHostToDevice) != hipSuccess ||
+        (p->nchunks && (hipMemcpy(p->lo, ch.lo.data(), p->nchunks * 8, hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(p->hi, ch.hi.data(), p->nchunks * 8, hipMemcpyHostToDevice) != hipSuccess)) ||
+        hipMemcpy(p->seg, ch.seg.data(), ch.seg.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+      rc = set_error(BTE_EDEVICE, "btx plan: upload failed");
+      break;
+    }
+    if ((rc = fft_warm(p->m, false, false, s)) || (rc = fft_warm(p->m, true, true, s))) break;
+  } while (0);
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return BTE_OK;
+}
+
+// Kernel launches only (no allocation, copy or synchronisation).
+static int plan_enqueue(bte_btx_plan* p, const bte_p1_affine* ct1, const bte_p1* shares, const bte_fp12* ct2,
+                        bte_fp12* messages_out, hipStream_t s) {
+  const size_t B = p->B, K = p->K, T = p->T, m = p->m, M = m * K, S = B * K;
+  hipLaunchKernelGGL(k_g1_pad, dim3(blocks_for(M, 256)), dim3(256), 0, s, (const g1a*)ct1,
+                     (const uint8_t*)p->valid, B, m, K, p->c);
+  BTE_TRY(bte_fft_g1_batched((bte_p1*)p->c, m, K, 0, 0, s));
+  BTE_TRY(bte_g1_batch_normalize((const bte_p1*)p->c, M, (bte_p1_affine*)p->ca, s));
+  BTE_TRY(bte_miller_loop_prepared_tiled(p->lines, m, (const bte_p1_affine*)p->ca, M, (bte_fp12*)p->ml, s));
+  BTE_TRY(bte_final_exp_easy((const bte_fp12*)p->ml, M, (bte_fp12*)p->conv, s));
+  BTE_TRY(cyc_prefix_batched_enqueue(p->conv, m, K, B, p->cyctab, s));
+  hipLaunchKernelGGL(k_gather_prefix, dim3(blocks_for(S, 256)), dim3(256), 0, s, (const fp12*)p->conv, m, B, K,
+                     p->pref);
+  BTE_TRY(bte_final_exp_hard((const bte_fp12*)p->pref, S, (bte_fp12*)p->beta, s));
+  BTE_TRY(bte_g1_batch_normalize(shares, K * T, (bte_p1_affine*)p->sa, s));
+  hipLaunchKernelGGL(k_open_pairs_prepared, dim3(blocks_for(S * T, 256)), dim3(256), 0, s, (const g1a*)p->sa,
+                     p->key_lines, T, B, K, p->P, p->tab);
+  BTE_TRY(mm_enqueue((const void* const*)p->tab, p->P, p->lo, p->hi, p->nchunks, p->seg, S, p->part, p->mm, s));
+  BTE_TRY(bte_final_exp((const bte_fp12*)p->mm, S, (bte_fp12*)p->alpha, s));
+  hipLaunchKernelGGL(k_open_combine, dim3(blocks_for(S, 64)), dim3(64), 0, s, (const fp12*)ct2,
+                     (const fp12*)p->alpha, (const fp12*)p->beta, (const uint8_t*)p->valid, S, (fp12*)messages_out);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+extern "C" int bte_btx_plan_run(bte_btx_plan* plan, const bte_p1_affine* ct1, const bte_p1* shares,
+                                const bte_fp12* ct2, bte_fp12* messages_out, void* stream) {
+  if (!plan || !ct1 || !shares || !ct2 || !messages_out) return set_error(BTE_EINVAL, "btx_plan_run: null pointer");
+  return plan_enqueue(plan, ct1, shares, ct2, messages_out, (hipStream_t)stream);
+}
+
+extern "C" int bte_btx_plan_launch_graph(bte_btx_plan* plan, const bte_p1_affine* ct1, const bte_p1* shares,
+                                         const bte_fp12* ct2, bte_fp12* messages_out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!plan || !ct1 || !shares || !ct2 || !messages_out) return set_error(BTE_EINVAL, "btx_plan_graph: null pointer");
+  if (!s) return set_error(BTE_EINVAL, "btx_plan_graph: the null stream cannot be captured");
+  const void* args[4] = {ct1, shares, ct2, messages_out};
+  if (!plan->exec) {
+    if (PhaseTimer(s).on)  // phase timing synchronises the stream, which a capture cannot contain
+      return set_error(BTE_EINVAL, "btx_plan_graph: disable profiling before the first launch");
+    BTE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = plan_enqueue(plan, ct1, shares, ct2, messages_out, s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return set_error(BTE_EDEVICE, "btx_plan_graph: capture failed: %s", hipGetErrorString(e));
+    plan->graph = g;
+    BTE_HIP(hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0));
+    for (int i = 0; i < 4; ++i) plan->bound[i] = args[i];
+  } else {
+    for (int i = 0; i < 4; ++i)
+      if (plan->bound[i] != args[i])
+        return set_error(BTE_EINVAL, "btx_plan_graph: the graph is bound to the buffers of its first launch");
+  }
+  BTE_HIP(hipGraphLaunch(plan->exec, s));
+  return BTE_OK;
+}
+
+extern "C" int bte_btx_plan_destroy(bte_btx_plan* plan) {
+  delete plan;
   return BTE_OK;
 }

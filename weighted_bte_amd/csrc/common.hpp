@@ -59,11 +59,30 @@ int miller_prepared_segmented(const void* const* tables, const uint64_t* seg_off
 // HOST segment offsets; pairs of a segment share accumulators in chunks (pairing.hip).
 int multi_miller_prepared(const void* const* pair_tables, const void* p, size_t npairs, const uint64_t* seg_offsets,
                           size_t nseg, void* out, hipStream_t s);
+// Same with segment s = pairs [lo[s], hi[s]) (HOST arrays; ranges may leave gaps).
+int multi_miller_prepared_ranges(const void* const* pair_tables, const void* p, const uint64_t* lo,
+                                 const uint64_t* hi, size_t nseg, void* out, hipStream_t s);
+// The two halves of it: host chunking, then launches over device copies of the chunk arrays
+// (part: one Fp12 per chunk) with no host work, for plans and graph capture.
+#include <vector>
+struct MMChunks {
+  std::vector<uint64_t> lo, hi, seg;  // chunk ranges; chunks of segment s are seg[s] .. seg[s+1]
+};
+void mm_chunks(const uint64_t* lo, const uint64_t* hi, size_t nseg, MMChunks& out);
+int mm_enqueue(const void* const* tables, const void* p, const uint64_t* d_lo, const uint64_t* d_hi, size_t nchunks,
+               const uint64_t* d_seg, size_t nseg, void* part, void* out, hipStream_t s);
 
 // Cyclotomic powers over K batches of B values with one exponent per slot i (element b B + i):
 // canonical scalars (HOST, B x 32 bytes) or small signed integers (device, B) (fft.hip).
 int cyc_pow_scalars_tiled(fp12* v, const uint8_t* host_scalars, size_t B, size_t K, hipStream_t s);
 int cyc_pow_small_tiled(fp12* v, const int64_t* coefficients, size_t B, size_t K, hipStream_t s);
+
+// Launch-only pieces for plans and graph capture (fft.hip): odd-power storage of the three-lane
+// cyclotomic stages, the prefix inverse transform over `count` transforms with caller storage,
+// and domain tables built ahead of time.
+size_t cyc_tab_bytes(size_t n, size_t count);
+int cyc_prefix_batched_enqueue(fp12* v, size_t n, size_t count, size_t out_count, uint32_t* tab, hipStream_t s);
+int fft_warm(size_t n, bool inverse, bool plans, hipStream_t s);
 
 // Segment holding flat index g: off[f] <= g < off[f + 1], off[0] = 0, count segments.
 __device__ inline uint32_t seg_of(const uint64_t* __restrict__ off, uint32_t count, uint64_t g) {

@@ -363,9 +363,17 @@ static int group_fft(jac<F>* v, size_t n, size_t count, bool inverse, bool scale
   return BTE_OK;
 }
 
-// mode 0: full transform; mode 1: prefix inverse (last stage only for `out_count` outputs)
+// Bytes of per-lane odd-power storage the cyclotomic stages of `count` transforms of size n
+// need (zero when they run one thread per butterfly).
+size_t cyc_tab_bytes(size_t n, size_t count) {
+  const size_t nbf = n / 2 * count;
+  return nbf && nbf < C3_MAX_BUTTERFLIES ? (size_t)c3_blocks(nbf) * 64 * 768 : 0;
+}
+
+// mode 0: full transform; mode 1: prefix inverse (last stage only for `out_count` outputs).
+// ext_tab: caller-owned storage of cyc_tab_bytes(n, count) bytes, or null to allocate here.
 static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool scaled, size_t out_count, int mode,
-                          hipStream_t s) {
+                          hipStream_t s, uint32_t* ext_tab = nullptr) {
   const int logn = log2_exact(n);
   if (logn < 0) return set_error(BTE_EINVAL, "fft: InvalidBatchSize(%zu)", n);
   if (count == 0) return BTE_OK;
@@ -377,8 +385,8 @@ static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool sc
   if (rc) return rc;
   const size_t nbf = n / 2 * count;
   const bool three = nbf < C3_MAX_BUTTERFLIES;
-  uint32_t* tab = nullptr;  // odd powers of the plan exponentiations, 768 bytes per lane
-  if (three) BTE_HIP(hipMallocAsync((void**)&tab, (size_t)c3_blocks(nbf) * 64 * 768, s));
+  uint32_t* tab = ext_tab;  // odd powers of the plan exponentiations, 768 bytes per lane
+  if (three && !tab) BTE_HIP(hipMallocAsync((void**)&tab, (size_t)c3_blocks(nbf) * 64 * 768, s));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse<fp12>), dim3(blocks_for(n * count, 256)), dim3(256), 0, s, v, n,
                      logn, count);
   const int full_stages = mode == 1 ? logn - 1 : logn;
@@ -398,7 +406,7 @@ static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool sc
       hipLaunchKernelGGL(k_cyc_prefix_last, dim3(blocks_for(out_count * count, 64)), dim3(64), 0, s, v, n,
                          out_count, (const cyc_plan*)t->plans, count);
   }
-  if (tab) (void)hipFreeAsync(tab, s);
+  if (tab && !ext_tab) (void)hipFreeAsync(tab, s);
   if (mode == 0 && inverse && scaled)
     hipLaunchKernelGGL(k_cyc_scale, dim3(blocks_for(n * count, 64)), dim3(64), 0, s, v, n * count,
                        (const cyc_plan*)t->size_inv_plan);
@@ -407,6 +415,20 @@ static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool sc
 }
 
 // ---------------------------------------------------------------------------------------------
+int cyc_prefix_batched_enqueue(fp12* v, size_t n, size_t count, size_t out_count, uint32_t* tab,
+                                    hipStream_t s) {
+  if (cyc_tab_bytes(n, count) && !tab) return set_error(BTE_EINVAL, "cyclotomic prefix: missing odd-power storage");
+  return cyclotomic_fft(v, n, count, true, false, out_count, 1, s, tab);
+}
+
+// Builds the twiddle tables (and cyclotomic plans) of a domain ahead of launch-only use.
+int fft_warm(size_t n, bool inverse, bool plans, hipStream_t s) {
+  const int logn = log2_exact(n);
+  if (logn < 0) return set_error(BTE_EINVAL, "fft: InvalidBatchSize(%zu)", n);
+  DomainTables* t;
+  return domain_tables(logn, inverse, plans, &t, s);
+}
+
 // Segmented transforms: `count` transforms of uneven power-of-two sizes stored back to back.
 // Stage lh (butterfly size 2^(lh+1)) uses twiddles w_{2h}^k, which do not depend on the
 // transform size, so one table for the largest size serves all: index k << (logmax-1-lh).

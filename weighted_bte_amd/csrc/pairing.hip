@@ -165,15 +165,16 @@ __global__ void __launch_bounds__(64) k_miller_pairs(const g1a* __restrict__ p, 
 
 // Multi-Miller loop over prepared tables with a shared accumulator (the squarings of
 // blst_miller_loop_n shared by the pairs of a chunk): thread j runs pairs
-// [chunk_off[j], chunk_off[j+1]) of one segment, pair g against table tables[g] and point P[g];
+// [chunk_lo[j], chunk_hi[j]) of one segment, pair g against table tables[g] and point P[g];
 // pairs with an identity are skipped (final_exponentiation.rs:238-256).  out[j] is conjugated.
 __global__ void __launch_bounds__(64) k_multi_miller_prepared(const line_t* const* __restrict__ tables,
                                                               const g1a* __restrict__ P,
-                                                              const uint64_t* __restrict__ chunk_off, size_t nchunks,
+                                                              const uint64_t* __restrict__ chunk_lo,
+                                                              const uint64_t* __restrict__ chunk_hi, size_t nchunks,
                                                               fp12* __restrict__ out) {
   const size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (j >= nchunks) return;
-  const uint64_t a = chunk_off[j], b = chunk_off[j + 1];
+  const uint64_t a = chunk_lo[j], b = chunk_hi[j];
   fp12 f = fp12_one();
   int k = 0;
 #pragma unroll 1
@@ -322,36 +323,68 @@ extern "C" int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine
 // (one wave per SIMD), since the shared squarings save less than idle SIMDs cost.
 static constexpr uint64_t MM_CHUNK_MAX = 8, MM_MIN_THREADS = 1 << 16;
 
-int bte::multi_miller_prepared(const void* const* pair_tables, const void* p, size_t npairs, const uint64_t* seg_offsets,
-                          size_t nseg, void* out, hipStream_t s) {
-  if (nseg == 0) return BTE_OK;
+void bte::mm_chunks(const uint64_t* lo, const uint64_t* hi, size_t nseg, MMChunks& out) {
+  uint64_t npairs = 0;
+  for (size_t g = 0; g < nseg; ++g) npairs += hi[g] - lo[g];
   const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(MM_CHUNK_MAX, npairs / MM_MIN_THREADS));
-  std::vector<uint64_t> chunk_off{0}, seg_chunks{0};
+  out.lo.clear();
+  out.hi.clear();
+  out.seg.assign(1, 0);
   for (size_t g = 0; g < nseg; ++g) {
-    const uint64_t a = seg_offsets[g], b = seg_offsets[g + 1];
-    if (b < a || b > npairs) return set_error(BTE_EINVAL, "multi_miller_prepared: bad segment offsets");
-    for (uint64_t c = a; c < b; c += chunk) chunk_off.push_back(std::min(b, c + chunk));
-    seg_chunks.push_back(chunk_off.size() - 1);
+    for (uint64_t c = lo[g]; c < hi[g]; c += chunk) {
+      out.lo.push_back(c);
+      out.hi.push_back(std::min(hi[g], c + chunk));
+    }
+    out.seg.push_back(out.lo.size());
   }
-  const size_t nchunks = chunk_off.size() - 1;
-  uint64_t *d_chunk, *d_segc;
-  fp12* part;
-  BTE_HIP(hipMallocAsync((void**)&d_chunk, chunk_off.size() * 8, s));
-  BTE_HIP(hipMallocAsync((void**)&d_segc, seg_chunks.size() * 8, s));
-  BTE_HIP(hipMallocAsync((void**)&part, std::max<size_t>(nchunks, 1) * sizeof(fp12), s));
-  BTE_HIP(hipMemcpyAsync(d_chunk, chunk_off.data(), chunk_off.size() * 8, hipMemcpyHostToDevice, s));
-  BTE_HIP(hipMemcpyAsync(d_segc, seg_chunks.data(), seg_chunks.size() * 8, hipMemcpyHostToDevice, s));
-  BTE_HIP(hipStreamSynchronize(s));  // host vectors are temporaries
+}
+
+int bte::mm_enqueue(const void* const* tables, const void* p, const uint64_t* d_lo, const uint64_t* d_hi,
+                    size_t nchunks, const uint64_t* d_seg, size_t nseg, void* part, void* out, hipStream_t s) {
+  if (nseg == 0) return BTE_OK;
   if (nchunks)
     hipLaunchKernelGGL(k_multi_miller_prepared, dim3(blocks_for(nchunks, 64)), dim3(64), 0, s,
-                       (const line_t* const*)pair_tables, (const g1a*)p, (const uint64_t*)d_chunk, nchunks, part);
-  hipLaunchKernelGGL(k_segment_product, dim3((unsigned)nseg), dim3(64), 0, s, (const fp12*)part,
-                     (const uint64_t*)d_segc, nseg, (fp12*)out);
-  (void)hipFreeAsync(d_chunk, s);
-  (void)hipFreeAsync(d_segc, s);
-  (void)hipFreeAsync(part, s);
+                       (const line_t* const*)tables, (const g1a*)p, d_lo, d_hi, nchunks, (fp12*)part);
+  hipLaunchKernelGGL(k_segment_product, dim3((unsigned)nseg), dim3(64), 0, s, (const fp12*)part, d_seg, nseg,
+                     (fp12*)out);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
+}
+
+int bte::multi_miller_prepared_ranges(const void* const* pair_tables, const void* p, const uint64_t* lo,
+                                      const uint64_t* hi, size_t nseg, void* out, hipStream_t s) {
+  if (nseg == 0) return BTE_OK;
+  MMChunks ch;
+  mm_chunks(lo, hi, nseg, ch);
+  const size_t nchunks = ch.lo.size();
+  uint64_t *d_lo, *d_hi, *d_seg;
+  fp12* part;
+  const size_t nc = std::max<size_t>(nchunks, 1);
+  BTE_HIP(hipMallocAsync((void**)&d_lo, nc * 8, s));
+  BTE_HIP(hipMallocAsync((void**)&d_hi, nc * 8, s));
+  BTE_HIP(hipMallocAsync((void**)&d_seg, ch.seg.size() * 8, s));
+  BTE_HIP(hipMallocAsync((void**)&part, nc * sizeof(fp12), s));
+  if (nchunks) {
+    BTE_HIP(hipMemcpyAsync(d_lo, ch.lo.data(), nchunks * 8, hipMemcpyHostToDevice, s));
+    BTE_HIP(hipMemcpyAsync(d_hi, ch.hi.data(), nchunks * 8, hipMemcpyHostToDevice, s));
+  }
+  BTE_HIP(hipMemcpyAsync(d_seg, ch.seg.data(), ch.seg.size() * 8, hipMemcpyHostToDevice, s));
+  BTE_HIP(hipStreamSynchronize(s));  // host vectors are temporaries
+  int rc = mm_enqueue(pair_tables, p, d_lo, d_hi, nchunks, d_seg, nseg, part, out, s);
+  (void)hipFreeAsync(d_lo, s);
+  (void)hipFreeAsync(d_hi, s);
+  (void)hipFreeAsync(d_seg, s);
+  (void)hipFreeAsync(part, s);
+  return rc;
+}
+
+int bte::multi_miller_prepared(const void* const* pair_tables, const void* p, size_t npairs, const uint64_t* seg_offsets,
+                               size_t nseg, void* out, hipStream_t s) {
+  if (nseg == 0) return BTE_OK;
+  for (size_t g = 0; g < nseg; ++g)
+    if (seg_offsets[g + 1] < seg_offsets[g] || seg_offsets[g + 1] > npairs)
+      return set_error(BTE_EINVAL, "multi_miller_prepared: bad segment offsets");
+  return multi_miller_prepared_ranges(pair_tables, p, seg_offsets, seg_offsets + 1, nseg, out, s);
 }
 
 extern "C" int bte_multi_miller_loop_prepared(const bte_fp6* const* pair_tables, const bte_p1_affine* p, size_t npairs,
