@@ -264,6 +264,16 @@ int bte_btx_plan_launch_graph(bte_btx_plan* plan, const bte_p1_affine* ct1, cons
                               const bte_fp12* ct2, bte_fp12* messages_out, void* stream);
 int bte_btx_plan_destroy(bte_btx_plan* plan);
 
+/* weighted_btx/src/setup.rs:98-233 keygen_with_rng, with the dealer's random choices given:
+ * poly = n_coeffs canonical scalars (the degree-t Shamir polynomial, t = n_coeffs - 1), q = N
+ * nonzero canonical scalars, party_offsets = N+1 prefix sums of the weights (all HOST).
+ * Outputs (device): material = negative_material (L blocks of W) then positive_material (L-1
+ * blocks of W), G2 affine, as bte_btx_prepare reads it; verification_keys = L x N G2 affine
+ * ([q_j^(i+1)]_2 at i N + j) or NULL. */
+int bte_btx_keygen(const uint8_t* poly, size_t n_coeffs, const uint8_t* q, const uint64_t* party_offsets,
+                   size_t n_parties, size_t max_batch, bte_p2_affine* material, bte_p2_affine* verification_keys,
+                   void* stream);
+
 /* decryption.rs:205-233 partial_decrypt for N parties at once: sigma_j = sum_i q_j^i ct1_i,
  * i = 1..B.  q = N canonical scalars (HOST); sigma_out = N G1 Jacobian (device). */
 int bte_btx_partial_decrypt(const bte_p1_affine* ct1, size_t batch_size, const uint8_t* q,

@@ -228,3 +228,23 @@ def test_btx_plan_run_and_graph_replays(api, B, K):
         plan.close()
     one = O.f12_to_bytes(O.F12_ONE)
     assert want[576:1152] == one
+
+
+def test_gpu_keygen_equals_host_tables(api):
+    """Device key generation (Horner evaluation, power tables, fixed-base G2 multiplications)
+    gives byte-identical material to the host-computed scalar tables, and the verification keys
+    are [q_j^(i+1)]_2."""
+    import torch
+
+    from weighted_bte_amd import inputs, workloads
+
+    weights, L = [2, 1, 3, 1], 4
+    wl = workloads.BtxWorkload(weights, threshold=3, max_batch=L, seed=41)
+    poly = inputs.Stream(41, 4000).scalars(4)
+    host = wl.host_material(poly)
+    assert torch.equal(wl.material, host)
+    mat, vk = api.btx_keygen(poly, wl.q, wl.offsets, L, verification_keys=True)
+    assert torch.equal(mat, host)
+    pows = [pow(q, i + 1, O.R) for i in range(L) for q in wl.q]
+    want = api.g2_generator_mul(api.to_device(b"".join(p.to_bytes(32, "little") for p in pows)))
+    assert torch.equal(vk, want)

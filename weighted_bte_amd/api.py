@@ -477,6 +477,20 @@ def btx_open_prepared_batched(shares_jac, n_accepted, key_lines, ct2, beta, vali
     return out
 
 
+def btx_keygen(poly, q, party_offsets, max_batch, verification_keys=False):
+    """setup.rs:98-233 key material from the dealer's choices (polynomial coefficients and party
+    q's as canonical ints): returns the device G2 material (negative then positive blocks) and,
+    if asked, the L x N verification keys."""
+    N, L = len(q), max_batch
+    W = party_offsets[-1]
+    mat = torch.empty((2 * L - 1) * W * G2_AFFINE, dtype=torch.uint8, device="cuda")
+    vk = torch.empty(L * N * G2_AFFINE, dtype=torch.uint8, device="cuda") if verification_keys else None
+    offs = (ctypes.c_uint64 * len(party_offsets))(*party_offsets)
+    call("bte_btx_keygen", _host_scalars(poly), len(poly), _host_scalars(q), offs, N, L, _ptr(mat), _ptr(vk),
+         _stream())
+    return (mat, vk) if verification_keys else mat
+
+
 class BtxPlan:
     """Launch-only plan of the fused decrypt step for `count` batches of B ciphertexts of one
     committee (kernel_lines / key_lines from btx_prepare and prepare_g2_lines(keys)); run()

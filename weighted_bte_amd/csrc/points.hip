@@ -76,29 +76,35 @@ template <class F> BTE_DEV aff<F> generator();
 template <> BTE_DEV g1a generator<fp>() { return g1_generator(); }
 template <> BTE_DEV g2a generator<fp2>() { return g2_generator(); }
 
-// table[i] = 2^i * G, i = 0..255 (one thread; run once per device).
+// Byte-window table: table[256 w + j] = j 2^(8w) G for w < 32, j < 256 (entry 0 of each window is
+// the identity).  One thread per window builds its 256 multiples; run once per device.
+static constexpr int GEN_WINDOWS = 32, GEN_ENTRIES = 256;
+
 template <class F> __global__ void k_generator_table(jac<F>* table) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  jac<F> p = jac_from_aff(generator<F>());
-  for (int i = 0; i < 256; ++i) {
-    table[i] = p;
-    p = jac_dbl(p);
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= GEN_WINDOWS) return;
+  jac<F> base = jac_from_aff(generator<F>());
+  for (int i = 0; i < 8 * w; ++i) base = jac_dbl(base);
+  jac<F> acc = jac_identity<F>();
+  for (int j = 0; j < GEN_ENTRIES; ++j) {
+    table[w * GEN_ENTRIES + j] = acc;
+    acc = jac_add(acc, base);
   }
 }
 
-// out[i] = k_i * G as the sum of the table entries of the set bits of k_i (full 256 bits).
+// out[i] = k_i * G as the sum of one table entry per nonzero byte of k_i (full 256 bits).
 template <class F>
 __global__ void __launch_bounds__(64) k_fixed_base_mul(const aff<F>* __restrict__ table, const uint32_t* __restrict__ scalars,
                                  size_t n, jac<F>* __restrict__ out) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   jac<F> acc = jac_identity<F>();
-  for (int w = 0; w < 8; ++w) {
-    uint32_t k = scalars[8 * i + w];
-    while (k) {
-      int b = __builtin_ctz(k);
-      k &= k - 1;
-      acc = jac_add_aff(acc, table[w * 32 + b]);
+  for (int word = 0; word < 8; ++word) {
+    const uint32_t k = scalars[8 * i + word];
+#pragma unroll 1
+    for (int b = 0; b < 4; ++b) {
+      const uint32_t byte = (k >> (8 * b)) & 0xffu;
+      if (byte) acc = jac_add_aff(acc, table[(4 * word + b) * GEN_ENTRIES + byte]);
     }
   }
   out[i] = acc;
@@ -118,10 +124,11 @@ template <class F> static int generator_table(const aff<F>** out, hipStream_t s)
   if (!g.table[dev]) {
     jac<F>* tj = nullptr;
     aff<F>* ta = nullptr;
-    BTE_HIP(hipMalloc((void**)&tj, 256 * sizeof(jac<F>)));
-    BTE_HIP(hipMalloc((void**)&ta, 256 * sizeof(aff<F>)));
+    const size_t nt = (size_t)GEN_WINDOWS * GEN_ENTRIES;
+    BTE_HIP(hipMalloc((void**)&tj, nt * sizeof(jac<F>)));
+    BTE_HIP(hipMalloc((void**)&ta, nt * sizeof(aff<F>)));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_generator_table<F>), dim3(1), dim3(64), 0, s, tj);
-    int rc = batch_normalize<F>(tj, 256, ta, s);
+    int rc = batch_normalize<F>(tj, nt, ta, s);
     if (rc) return rc;
     BTE_HIP(hipStreamSynchronize(s));
     BTE_HIP(hipFree(tj));

@@ -1,8 +1,8 @@
 """Synthetic weighted_btx workloads on the device (bench and tests).
 
-Restates the algebra of the reference's trusted-dealer setup (weighted_btx/src/setup.rs:98-233)
-and encryption (encryption.rs:28-49) with seeded scalars; group elements come from this
-library's fixed-base kernels and GT values from its pairing, so nothing here depends on a CPU
+The trusted-dealer setup (weighted_btx/src/setup.rs:98-233) runs on the device
+(api.btx_keygen) from seeded dealer choices; encryption (encryption.rs:28-49) uses seeded
+scalars, the library's fixed-base kernels and its pairing, so nothing here depends on a CPU
 implementation of the curve.
 """
 from __future__ import annotations
@@ -52,7 +52,7 @@ class BtxWorkload:
     """Key material for `weights`, threshold and max batch L, with device-resident G2 material
     laid out as the C ABI expects (negative blocks then positive blocks)."""
 
-    def __init__(self, weights, threshold, max_batch, seed=1):
+    def __init__(self, weights, threshold, max_batch, seed=1, gpu_keygen=True):
         st = inputs.Stream(seed, 4000)
         self.weights = list(weights)
         self.N = len(weights)
@@ -62,21 +62,31 @@ class BtxWorkload:
         self.offsets = [0]
         for w in weights:
             self.offsets.append(self.offsets[-1] + w)
-        owner = np.repeat(np.arange(self.N), weights)
+        poly = st.scalars(threshold + 1)
+        self.z = poly[0]
+        self.q = [q or 1 for q in st.scalars(self.N)]
+        self.stream = st
+        self.gt = gt_generator_bytes()
+        if gpu_keygen:
+            self.material = api.btx_keygen(poly, self.q, self.offsets, self.L)
+        else:
+            self.material = self.host_material(poly)
+
+    def host_material(self, poly):
+        """The same material with the scalar tables computed on the host (the check of the
+        device key generation)."""
+        owner = np.repeat(np.arange(self.N), self.weights)
         dom = 1
         while dom < self.W:
             dom <<= 1
         g = pow(ROOT_OF_UNITY, (1 << 32) // dom, R)
         xs = [pow(g, v, R) for v in range(self.W)]
-        poly = st.scalars(threshold + 1)
-        self.z = poly[0]
         zv = []
         for x in xs:  # Horner
             acc = 0
             for c in reversed(poly):
                 acc = (acc * x + c) % R
             zv.append(acc)
-        self.q = [q or 1 for q in st.scalars(self.N)]
         qinv = [pow(q, -1, R) for q in self.q]
         neg = bytearray()
         pos = bytearray()
@@ -88,9 +98,7 @@ class BtxWorkload:
         for i in range(1, self.L):
             pos += b"".join((zv[v] * qp[owner[v]] % R).to_bytes(32, "little") for v in range(self.W))
             qp = [a * b % R for a, b in zip(qp, self.q)]
-        self.material = api.g2_generator_mul(api.to_device(bytes(neg + pos)))
-        self.stream = st
-        self.gt = gt_generator_bytes()
+        return api.g2_generator_mul(api.to_device(bytes(neg + pos)))
 
     def encrypt(self, B, count=1):
         """count batches of B ciphertexts: messages gT^s, ct1 = r G1, ct2 = gT^(s + z r)."""
