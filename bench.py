@@ -414,12 +414,13 @@ def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batche
     }
 
 
-def bench_serving(n_requests=48, rate=40.0, burst_requests=256, burst_rate=2000.0, max_batch=256, seed=7):
+def bench_serving(n_requests=48, rate=40.0, burst_requests=256, burst_rate=2000.0, max_batch=256, seed=7,
+                  max_weight=2048, weight_per_party=4):
     """Poisson arrivals of batch-decryption requests (B log-uniform in 16..max_batch) against
-    three resident committees (n log-uniform in 16..1000 scaled to W <= 2048, Pareto(1.16)
-    weights), admitted by HBM budget, fused across committees and batch sizes.  Two offered
-    loads: `rate` (latency at moderate load) and `burst_rate` over `burst_requests` requests
-    (throughput at saturation)."""
+    three resident committees (n log-uniform in 16..1000, Pareto(1.16) weights summing to
+    W = min(max_weight, weight_per_party * n)), admitted by HBM budget, fused across committees
+    and batch sizes.  Two offered loads: `rate` (latency at moderate load) and `burst_rate` over
+    `burst_requests` requests (throughput at saturation)."""
     import dataclasses
     import math
     import random
@@ -433,7 +434,7 @@ def bench_serving(n_requests=48, rate=40.0, burst_requests=256, burst_rate=2000.
     wls, coms = [], []
     for k in range(3):
         n = int(round(math.exp(rnd.uniform(math.log(16), math.log(1000)))))
-        W = min(2048, max(n, 4 * n))
+        W = min(max_weight, max(n, weight_per_party * n))
         weights = workloads.pareto_weights(n, W, seed=seed + k)
         wl = workloads.BtxWorkload(weights, W // 2, max_batch, seed=seed + 10 * k)
         committee = workloads.heaviest_first(weights, wl.t)
@@ -450,9 +451,10 @@ def bench_serving(n_requests=48, rate=40.0, burst_requests=256, burst_rate=2000.
         reqs.append(serving.Request(rid, cid, B, ct1, ct2, shares))
         expected.append(msgs)
     for com in coms:  # committee preparation happens before traffic (resident state)
-        for bp in (16, 32, 64, 128, 256):
-            if bp <= max_batch:
-                com.prepare(bp)
+        bp = 16
+        while bp <= max_batch:
+            com.prepare(bp)
+            bp *= 2
     torch.cuda.synchronize()
     t_setup = time.time() - t0
 
@@ -469,6 +471,7 @@ def bench_serving(n_requests=48, rate=40.0, burst_requests=256, burst_rate=2000.
     moderate = run(n_requests, rate)
     burst = run(burst_requests, burst_rate) if burst_requests else None
     out = {"workload": "weighted_btx serving (Poisson arrivals, continuous batching)",
+           "max_weight": max_weight,
            "committees": len(coms), "committee_sizes": [len(c.accepted) for c in coms],
            "total_weights": [c.total_weight for c in coms], "max_batch": max_batch,
            "setup_s": round(t_setup, 1), "saturated": burst}
@@ -488,6 +491,8 @@ def main():
     ap.add_argument("--no-config2", action="store_true",
                     help="skip the MSM size table, the FFT size table and the uneven multi-pairing")
     ap.add_argument("--no-pfe", action="store_true", help="skip the weighted_pfe decryption measurement")
+    ap.add_argument("--no-serving-large", action="store_true",
+                    help="skip serving committees of up to 2^14 total weight with batches up to 2048")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
     ap.add_argument("--serving-requests", type=int, default=48,
@@ -580,6 +585,9 @@ def main():
     cpu = None if a.no_cpu else cpu_baseline(a.logn)
     decrypt = bench_decrypt(a.decrypt_batches, with_cpu=not a.no_cpu) if a.decrypt_batches > 0 else None
     serving = bench_serving(a.serving_requests) if a.serving_requests > 0 else None
+    serving_large = (bench_serving(a.serving_requests, burst_requests=64, max_batch=2048, max_weight=1 << 14,
+                                   weight_per_party=16) if a.serving_requests > 0 and not a.no_serving_large
+                     else None)
 
     line = {
         "metric": "G1 MSM 2^%d ms (decrypted ctxts/s at B=512 reported separately)" % a.logn,
@@ -622,13 +630,15 @@ def main():
         "pfe_decrypt": pfe,
         "decrypt": decrypt,
         "serving": serving,
+        "serving_large_committees": serving_large,
     }
     print(json.dumps(line))
     ok = parity and (decrypt is None or decrypt["plaintexts_bit_exact"]) and \
         (config2 is None or all(r["parity"] for r in config2)) and \
         (ffts is None or all(r["roundtrip_ok"] for r in ffts)) and \
         (mpair is None or mpair["parity"]) and (pfe is None or pfe["plaintexts_bit_exact"]) and \
-        (serving is None or serving["plaintexts_bit_exact"])
+        (serving is None or serving["plaintexts_bit_exact"]) and \
+        (serving_large is None or serving_large["plaintexts_bit_exact"])
     sys.exit(0 if ok else 1)
 
 
