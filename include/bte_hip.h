@@ -264,6 +264,22 @@ int bte_btx_plan_launch_graph(bte_btx_plan* plan, const bte_p1_affine* ct1, cons
                               const bte_fp12* ct2, bte_fp12* messages_out, void* stream);
 int bte_btx_plan_destroy(bte_btx_plan* plan);
 
+/* weighted_btx/src/decryption.rs:236-272 verify_decryption_share for n_shares shares at once:
+ * ok_out[j] (HOST) = 1 when prod_{valid i} e(ct1_i, vk[i N + party_index[j]]) e(-sigma_j, G2) is
+ * one.  shares (T G1 Jacobian), verification_keys (L x N G2 affine, bte_btx_keygen's layout) and
+ * ct1 (B G1 affine) are DEVICE; party_index (T) and valid (B) are HOST. */
+int bte_btx_verify_shares(const bte_p1* shares, const uint32_t* party_index, size_t n_shares,
+                          const bte_p2_affine* verification_keys, size_t n_parties, size_t max_batch,
+                          const bte_p1_affine* ct1, const uint8_t* valid, size_t batch_size, uint8_t* ok_out,
+                          void* stream);
+/* decryption.rs:356-422 verify_decryption_shares_batched: the shares folded with caller-chosen
+ * nonzero challenges (T canonical scalars, HOST): one G1 MSM, one batched indexed G2 MSM over
+ * the valid slots and one multi-pairing; *result (HOST) = 1 when the folded check holds. */
+int bte_btx_verify_shares_batched(const bte_p1* shares, const uint32_t* party_index, const uint8_t* challenges,
+                                  size_t n_shares, const bte_p2_affine* verification_keys, size_t n_parties,
+                                  size_t max_batch, const bte_p1_affine* ct1, const uint8_t* valid,
+                                  size_t batch_size, int* result, void* stream);
+
 /* weighted_btx/src/setup.rs:98-233 keygen_with_rng, with the dealer's random choices given:
  * poly = n_coeffs canonical scalars (the degree-t Shamir polynomial, t = n_coeffs - 1), q = N
  * nonzero canonical scalars, party_offsets = N+1 prefix sums of the weights (all HOST).

@@ -248,3 +248,38 @@ def test_gpu_keygen_equals_host_tables(api):
     pows = [pow(q, i + 1, O.R) for i in range(L) for q in wl.q]
     want = api.g2_generator_mul(api.to_device(b"".join(p.to_bytes(32, "little") for p in pows)))
     assert torch.equal(vk, want)
+
+
+def test_share_verification_each_and_batched(api):
+    """Valid shares pass both checks; a share moved by G1 fails, the batched check fails and
+    the per-share check blames exactly that party (decryption.rs:236-422)."""
+    import random as rnd_mod
+
+    import torch
+
+    from weighted_bte_amd import inputs, workloads
+
+    weights, L, B = [3, 1, 2, 4, 2], 8, 8
+    wl = workloads.BtxWorkload(weights, threshold=6, max_batch=L, seed=51)
+    poly = inputs.Stream(51, 4000).scalars(7)
+    _, vk = api.btx_keygen(poly, wl.q, wl.offsets, L, verification_keys=True)
+    ct1, ct2, msgs = wl.encrypt(B)
+    valid = [True] * B
+    valid[3] = False
+    raw = bytearray(api.to_host(ct1))
+    raw[3 * 96:4 * 96] = bytes(96)
+    ct1 = api.to_device(bytes(raw))
+    sig = api.btx_partial_decrypt(ct1, wl.q)
+    parties = [0, 2, 3, 4]
+    shares = torch.cat([sig[144 * j:144 * (j + 1)] for j in parties])
+    rnd = rnd_mod.Random(5)
+    rho = [rnd.randrange(1, O.R) for _ in parties]
+    assert api.btx_verify_shares(shares, parties, vk, len(weights), L, ct1, valid) == [True] * 4
+    assert api.btx_verify_shares_batched(shares, parties, rho, vk, len(weights), L, ct1, valid)
+    # tamper with party 3's share: sigma + G1
+    bad = api.to_host(api.batch_normalize_g1(shares[2 * 144:3 * 144]))
+    bad_pt = O.G1.add(O.G1.from_affine(O.g1_affine_from_bytes(bad)), O.g1_gen_jac())
+    tampered = shares.clone()
+    tampered[2 * 144:3 * 144] = api.to_device(O.g1_jac_to_bytes(bad_pt))
+    assert api.btx_verify_shares(tampered, parties, vk, len(weights), L, ct1, valid) == [True, True, False, True]
+    assert not api.btx_verify_shares_batched(tampered, parties, rho, vk, len(weights), L, ct1, valid)

@@ -491,6 +491,32 @@ def btx_keygen(poly, q, party_offsets, max_batch, verification_keys=False):
     return (mat, vk) if verification_keys else mat
 
 
+def btx_verify_shares(shares_jac, party_index, verification_keys, n_parties, max_batch, ct1_affine, valid):
+    """verify_decryption_share (decryption.rs:236-272) for every share: a list of bools."""
+    T = len(party_index)
+    B = len(valid)
+    pidx = (ctypes.c_uint32 * max(T, 1))(*party_index)
+    v = (ctypes.c_uint8 * B)(*[1 if x else 0 for x in valid])
+    ok = (ctypes.c_uint8 * max(T, 1))()
+    call("bte_btx_verify_shares", _ptr(_dev(shares_jac)), pidx, T, _ptr(_dev(verification_keys)), n_parties,
+         max_batch, _ptr(_dev(ct1_affine)), v, B, ok, _stream())
+    return [bool(ok[j]) for j in range(T)]
+
+
+def btx_verify_shares_batched(shares_jac, party_index, challenges, verification_keys, n_parties, max_batch,
+                              ct1_affine, valid):
+    """verify_decryption_shares_batched (decryption.rs:356-422) with the given challenges."""
+    T = len(party_index)
+    B = len(valid)
+    pidx = (ctypes.c_uint32 * max(T, 1))(*party_index)
+    v = (ctypes.c_uint8 * B)(*[1 if x else 0 for x in valid])
+    res = ctypes.c_int(0)
+    call("bte_btx_verify_shares_batched", _ptr(_dev(shares_jac)), pidx, _host_scalars(challenges), T,
+         _ptr(_dev(verification_keys)), n_parties, max_batch, _ptr(_dev(ct1_affine)), v, B, ctypes.byref(res),
+         _stream())
+    return bool(res.value)
+
+
 class BtxPlan:
     """Launch-only plan of the fused decrypt step for `count` batches of B ciphertexts of one
     committee (kernel_lines / key_lines from btx_prepare and prepare_g2_lines(keys)); run()
