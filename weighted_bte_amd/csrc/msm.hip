@@ -39,13 +39,15 @@ namespace bte {
 static constexpr int MSM_CHUNK = 32;  // sorted entries per accumulation thread
 
 template <class F> struct MsmTraits;
+// Threads per window-sum workgroup: as many as the LDS tree fits (147 KB of Jacobian points),
+// so each (window, sum) has a short serial chain.
 template <> struct MsmTraits<fp> {
   using HF = bteh::Fp;
-  static constexpr int REDUCE_THREADS = 256;
+  static constexpr int REDUCE_THREADS = 1024;
 };
 template <> struct MsmTraits<fp2> {
   using HF = bteh::Fp2;
-  static constexpr int REDUCE_THREADS = 128;
+  static constexpr int REDUCE_THREADS = 512;
 };
 
 struct MsmShape {
@@ -271,28 +273,31 @@ __global__ void __launch_bounds__(64) k_msm_fixup(const uint32_t* __restrict__ o
   buckets[b] = acc;
 }
 
-// One 64-thread workgroup per queued bucket: strided partial sums, then a tree in LDS.  The grid
-// is sized for the most buckets that can span more than FIX_SERIAL chunks; surplus groups exit.
+// 64-thread workgroups take the queued buckets in turn (grid-stride over the device-side queue
+// length): strided partial sums, then a tree in LDS.
 template <class F>
 __global__ void __launch_bounds__(64) k_msm_fixup_big(const uint32_t* __restrict__ offsets,
                                                       jac<F>* __restrict__ buckets, const jac<F>* __restrict__ heads,
                                                       const jac<F>* __restrict__ tails, const uint32_t* __restrict__ big,
                                                       const uint32_t* __restrict__ nbig) {
   __shared__ jac<F> sh[64];
-  if (blockIdx.x >= *nbig) return;
-  const uint32_t b = big[blockIdx.x];
-  const uint32_t bs = offsets[b], be = offsets[b + 1];
-  const uint32_t cs = bs / MSM_CHUNK, ce = (be - 1) / MSM_CHUNK;
+  const uint32_t nq = *nbig;
   const int tid = threadIdx.x;
-  jac<F> acc = tid == 0 ? tails[cs] : jac_identity<F>();
-  for (uint32_t c = cs + 1 + tid; c <= ce; c += 64) acc = jac_add(acc, heads[c]);
-  sh[tid] = acc;
-  __syncthreads();
-  for (int w = 32; w > 0; w >>= 1) {
-    if (tid < w) sh[tid] = jac_add(sh[tid], sh[tid + w]);
+  for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {  // uniform per workgroup
+    const uint32_t b = big[qi];
+    const uint32_t bs = offsets[b], be = offsets[b + 1];
+    const uint32_t cs = bs / MSM_CHUNK, ce = (be - 1) / MSM_CHUNK;
+    jac<F> acc = tid == 0 ? tails[cs] : jac_identity<F>();
+    for (uint32_t c = cs + 1 + tid; c <= ce; c += 64) acc = jac_add(acc, heads[c]);
+    sh[tid] = acc;
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+      if (tid < w) sh[tid] = jac_add(sh[tid], sh[tid + w]);
+      __syncthreads();
+    }
+    if (tid == 0) buckets[b] = sh[0];
     __syncthreads();
   }
-  if (tid == 0) buckets[b] = sh[0];
 }
 
 template <class F>
@@ -561,7 +566,7 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup<F>), dim3(blocks_for(nb, 64)), dim3(64), 0, s,
                      (const uint32_t*)offsets, nb, buckets, (const jac<F>*)heads, (const jac<F>*)tails, big,
                      nbig);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_big<F>), dim3((unsigned)max_big), dim3(64), 0, s,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_big<F>), dim3((unsigned)std::min<size_t>(max_big, 2048)), dim3(64), 0, s,
                      (const uint32_t*)offsets, buckets, (const jac<F>*)heads, (const jac<F>*)tails,
                      (const uint32_t*)big, (const uint32_t*)nbig);
   pt.mark("fixup");
@@ -675,7 +680,7 @@ static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t
   }
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup<F>), dim3(blocks_for(nb, 64)), dim3(64), 0, s,
                      (const uint32_t*)offsets, nb, buckets, (const jac<F>*)heads, (const jac<F>*)tails, big, nbig);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_big<F>), dim3((unsigned)max_big), dim3(64), 0, s,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_big<F>), dim3((unsigned)std::min<size_t>(max_big, 2048)), dim3(64), 0, s,
                      (const uint32_t*)offsets, buckets, (const jac<F>*)heads, (const jac<F>*)tails,
                      (const uint32_t*)big, (const uint32_t*)nbig);
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_segsum<F>), dim3(blocks_for(nss, 64)), dim3(64), 0, s,
