@@ -57,6 +57,33 @@ def test_lagrange_matches_oracle_definition(api):
     assert api.lagrange_at_zero(16, list(range(16))) == [pow(16, -1, O.R)] * 16
 
 
+def test_lagrange_large_subset_interpolates(api):
+    """700 of the first 1000 points of the size-1024 domain (the device path for the n^2
+    denominators): the coefficients reproduce f(0) from f(x_v) for a random f of degree 699, and
+    spot coefficients equal the definition."""
+    rnd = random.Random(8)
+    W = 1000
+    sel = sorted(rnd.sample(range(W), 700))
+    got = api.lagrange_at_zero(W, sel)
+    dom = O.Radix2Domain(1024)
+    xs = [pow(dom.generator, v, O.R) for v in sel]
+    f = [rnd.randrange(O.R) for _ in range(700)]
+
+    def ev(x):
+        acc = 0
+        for c in reversed(f):
+            acc = (acc * x + c) % O.R
+        return acc
+    assert sum(l * ev(x) for l, x in zip(got, xs)) % O.R == f[0]
+    for i in (0, 349, 699):
+        num, den = 1, 1
+        for j, xj in enumerate(xs):
+            if j != i:
+                num = num * xj % O.R
+                den = den * (xj - xs[i]) % O.R
+        assert got[i] == num * pow(den, -1, O.R) % O.R
+
+
 @pytest.mark.parametrize("B", [1, 2, 3, 7, 8])
 def test_btx_beta_matches_literal_formula_and_decrypts(api, B):
     st = BtxSetup(api, [1, 2, 3], threshold=3, max_batch=8, seed=B)

@@ -14,6 +14,7 @@
 #include "common.hpp"
 #include "curve.hpp"
 #include "cyclo.hpp"
+#include "fr.hpp"
 #include "host_field.hpp"
 
 extern "C" {
@@ -44,6 +45,44 @@ namespace bte {
 // unique field values; this restatement uses L_v(0) = prod_{u != v} x_u / (x_u - x_v), with one
 // batched inversion, and the closed form 1/M for a complete domain (interpolation.rs:36-44).
 // ---------------------------------------------------------------------------------------------
+// den_v = x_v prod_{u != v} (x_u - x_v) (Montgomery in and out), one thread per point.
+__global__ void __launch_bounds__(256) k_lagrange_den(const fr* __restrict__ x, size_t n, fr* __restrict__ den) {
+  const size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const fr xv = x[v];
+  fr d = xv;
+  for (size_t u = 0; u < n; ++u)
+    if (u != v) d = fr_mont_mul(d, fr_sub(x[u], xv));
+  den[v] = d;
+}
+
+__global__ void k_fr_mont(fr* v, size_t n, int to) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) v[i] = to ? fr_to_mont(v[i]) : fr_from_mont(v[i]);
+}
+
+static constexpr size_t LAGRANGE_GPU_MIN = 512;
+
+// Synchronous (legacy default stream): the host interface of lagrange_at_zero has no stream.
+static int lagrange_denominators_gpu(const std::vector<bteh::Fr>& x, std::vector<bteh::Fr>& den) {
+  const size_t n = x.size();
+  fr *d_x = nullptr, *d_den = nullptr;
+  BTE_HIP(hipMalloc((void**)&d_x, n * sizeof(fr)));
+  hipError_t e = hipMalloc((void**)&d_den, n * sizeof(fr));
+  if (e == hipSuccess) e = hipMemcpy(d_x, x.data(), n * sizeof(fr), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_fr_mont, dim3(blocks_for(n, 256)), dim3(256), 0, 0, d_x, n, 1);
+    hipLaunchKernelGGL(k_lagrange_den, dim3(blocks_for(n, 256)), dim3(256), 0, 0, (const fr*)d_x, n, d_den);
+    hipLaunchKernelGGL(k_fr_mont, dim3(blocks_for(n, 256)), dim3(256), 0, 0, d_den, n, 0);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(den.data(), d_den, n * sizeof(fr), hipMemcpyDeviceToHost);
+  (void)hipFree(d_x);
+  if (d_den) (void)hipFree(d_den);
+  if (e != hipSuccess) return set_error(BTE_EDEVICE, "lagrange_at_zero: %s", hipGetErrorString(e));
+  return BTE_OK;
+}
+
 static int lagrange_at_zero(size_t total_weight, const std::vector<uint64_t>& sel, std::vector<bteh::Fr>& out) {
   using namespace bteh;
   const size_t n = sel.size();
@@ -74,11 +113,17 @@ static int lagrange_at_zero(size_t total_weight, const std::vector<uint64_t>& se
   Fr prod_all = fr_from_u64(1);
   for (size_t i = 0; i < n; ++i) prod_all = fr_mul(prod_all, x[i]);
   std::vector<Fr> den(n);
-  for (size_t v = 0; v < n; ++v) {
-    Fr d = x[v];  // folds the 1/x_v of num_v = prod_all / x_v into the denominator
-    for (size_t u = 0; u < n; ++u)
-      if (u != v) d = fr_mul(d, fr_sub(x[u], x[v]));
-    den[v] = d;
+  if (n >= LAGRANGE_GPU_MIN) {
+    // the n^2 products on the device, one thread per point
+    int rc = lagrange_denominators_gpu(x, den);
+    if (rc) return rc;
+  } else {
+    for (size_t v = 0; v < n; ++v) {
+      Fr d = x[v];  // folds the 1/x_v of num_v = prod_all / x_v into the denominator
+      for (size_t u = 0; u < n; ++u)
+        if (u != v) d = fr_mul(d, fr_sub(x[u], x[v]));
+      den[v] = d;
+    }
   }
   // batch inversion of den
   std::vector<Fr> pre(n);

@@ -86,4 +86,17 @@ BTE_DEV fr fr_add(const fr& a, const fr& b) {
   return fr_reduce_once(r);  // a, b < r < 2^255: no carry out of the top limb
 }
 
+BTE_DEV fr fr_sub(const fr& a, const fr& b) {
+  fr r;
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r.l[i] = __builtin_subc(a.l[i], b.l[i], borrow, &borrow);
+  if (borrow) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r.l[i] = __builtin_addc(r.l[i], fr_limb(i), c, &c);
+  }
+  return r;
+}
+
 }  // namespace bte
