@@ -88,6 +88,17 @@ int bte_g1_batch_normalize(const bte_p1* in, size_t n, bte_p1_affine* out, void*
 /* weighted_btx/src/blst_utils.rs:26-36 batch_normalize_g2 */
 int bte_g2_batch_normalize(const bte_p2* in, size_t n, bte_p2_affine* out, void* stream);
 
+/* Canonical compressed encodings (blstrs to_compressed / from_compressed: big-endian x, G2 as
+ * x.c1 || x.c0, flags 0x80 compressed, 0x40 infinity, 0x20 larger y).  compress: n points ->
+ * 48 n (G1) or 96 n (G2) bytes.  decompress: ok[i] = 1 when encoding i has valid flags, x < p,
+ * x on the curve and the point in the order-r subgroup (invalid elements decode to the
+ * identity).  All arrays DEVICE.  Wire formats used by weighted_pfe/src/encoding.rs:31-45 and
+ * the transcripts (weighted_pfe/src/proof.rs:62-65). */
+int bte_g1_compress(const bte_p1_affine* points, size_t n, uint8_t* out, void* stream);
+int bte_g2_compress(const bte_p2_affine* points, size_t n, uint8_t* out, void* stream);
+int bte_g1_decompress(const uint8_t* in, size_t n, bte_p1_affine* points, uint8_t* ok, void* stream);
+int bte_g2_decompress(const uint8_t* in, size_t n, bte_p2_affine* points, uint8_t* ok, void* stream);
+
 /* ---------------------------------------------------------------------------------------- *
  * a4-a9: radix-2 transforms over the Fr domain of size n (a power of two <= 2^32), in place.
  * Natural-order DFT X_k = sum_j x_j w^(jk), w = ROOT_OF_UNITY^(2^(32 - log n)); `inverse` uses
@@ -329,6 +340,14 @@ int bte_pfe_open_prepared_batched(const bte_p1* shares, size_t n_accepted, const
                                   const bte_p1* first_cauchy, const bte_fp12* pairing_cauchy,
                                   const uint8_t* alpha, size_t batch_size, size_t count,
                                   bte_fp12* messages_out, void* stream);
+
+/* weighted_pfe/src/proof.rs:74-122 verify_batch after the host-side transcript hashing: 1 when
+ * sum c_i s_i G - sum c_i A_i - sum c_i e_i C_i is the identity (one G1 MSM of 2B+1 terms).
+ * commitments A and ct1 C: B G1 affine (device); responses s, challenges e, coefficients c: B
+ * canonical scalars each (HOST); *result HOST. */
+int bte_pfe_verify_proofs_batched(const bte_p1_affine* commitments, const bte_p1_affine* ct1,
+                                  const uint8_t* responses, const uint8_t* challenges,
+                                  const uint8_t* coefficients, size_t batch_size, int* result, void* stream);
 
 /* decryption.rs:230-251 partial_decrypt for N parties: sigma_j = sum_i g_{j,i} ct1_i with the
  * N x B fractions g (canonical scalars, device). */

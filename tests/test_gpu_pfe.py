@@ -81,3 +81,24 @@ def test_pfe_fused_batches_equal_alone(api):
         alone = api.pfe_open(shares[b * T * 144:(b + 1) * T * 144], d1p, ml, c1, ct2[b * B * 576:(b + 1) * B * 576],
                              f1, p1, wl.alpha)
         assert api.to_host(alone) == api.to_host(fused[b * B * 576:(b + 1) * B * 576])
+
+
+def test_pfe_proof_batch_verification(api):
+    """Schnorr proofs s_i G = A_i + e_i C_i pass the folded one-MSM check; one wrong response
+    fails it (weighted_pfe/src/proof.rs:74-122)."""
+    import random
+
+    rnd = random.Random(12)
+    B = 16
+    r = [rnd.randrange(1, O.R) for _ in range(B)]
+    k = [rnd.randrange(1, O.R) for _ in range(B)]
+    e = [rnd.randrange(O.R) for _ in range(B)]
+    resp = [(ki + ei * ri) % O.R for ki, ei, ri in zip(k, e, r)]
+    coef = [rnd.randrange(1, O.R) for _ in range(B)]
+    pack = lambda v: api.to_device(b"".join(x.to_bytes(32, "little") for x in v))
+    ct1 = api.g1_generator_mul(pack(r))
+    commit = api.g1_generator_mul(pack(k))
+    assert api.pfe_verify_proofs_batched(commit, ct1, resp, e, coef)
+    bad = list(resp)
+    bad[5] = (bad[5] + 1) % O.R
+    assert not api.pfe_verify_proofs_batched(commit, ct1, bad, e, coef)

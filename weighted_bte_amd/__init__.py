@@ -81,6 +81,11 @@ _SIGNATURES = {
     "bte_btx_plan_destroy": [_vp],
     "bte_btx_keygen": [_vp, _sz, _vp, _vp, _sz, _sz, _vp, _vp, _vp],
     "bte_btx_verify_shares": [_vp, _vp, _sz, _vp, _sz, _sz, _vp, _vp, _sz, _vp, _vp],
+    "bte_pfe_verify_proofs_batched": [_vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp],
+    "bte_g1_compress": [_vp, _sz, _vp, _vp],
+    "bte_g2_compress": [_vp, _sz, _vp, _vp],
+    "bte_g1_decompress": [_vp, _sz, _vp, _vp, _vp],
+    "bte_g2_decompress": [_vp, _sz, _vp, _vp, _vp],
     "bte_btx_verify_shares_batched": [_vp, _vp, _vp, _sz, _vp, _sz, _sz, _vp, _vp, _sz, _vp, _vp],
     "bte_pfe_open_prepared_batched": [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _sz, _vp, _vp],
     "bte_multi_miller_loop_prepared": [_vp, _vp, _sz, _vp, _sz, _vp, _vp],

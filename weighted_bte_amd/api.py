@@ -102,6 +102,40 @@ def batch_normalize_g2(projective):
     return out
 
 
+# --- canonical compressed encodings (blstrs to_compressed / from_compressed) -----------------
+def g1_compress(points_affine):
+    p = _dev(points_affine)
+    n = p.numel() // G1_AFFINE
+    out = _empty(48 * n, p)
+    call("bte_g1_compress", _ptr(p), n, _ptr(out), _stream())
+    return out
+
+
+def g2_compress(points_affine):
+    p = _dev(points_affine)
+    n = p.numel() // G2_AFFINE
+    out = _empty(96 * n, p)
+    call("bte_g2_compress", _ptr(p), n, _ptr(out), _stream())
+    return out
+
+
+def g1_decompress(encodings):
+    """-> (affine points, validity flags as a uint8 tensor)."""
+    e = _dev(encodings)
+    n = e.numel() // 48
+    pts, ok = _empty(n * G1_AFFINE, e), _empty(n, e)
+    call("bte_g1_decompress", _ptr(e), n, _ptr(pts), _ptr(ok), _stream())
+    return pts, ok
+
+
+def g2_decompress(encodings):
+    e = _dev(encodings)
+    n = e.numel() // 96
+    pts, ok = _empty(n * G2_AFFINE, e), _empty(n, e)
+    call("bte_g2_decompress", _ptr(e), n, _ptr(pts), _ptr(ok), _stream())
+    return pts, ok
+
+
 # --- fixed-base generator multiples ----------------------------------------------------------
 def g1_generator_mul(scalar_bytes):
     scalar_bytes = _dev(scalar_bytes)
@@ -586,6 +620,16 @@ def pfe_prepare(material, total_weight, batch_size, party_offsets, accepted, alp
     call("bte_pfe_prepare", _ptr(material), total_weight, B, offs, len(party_offsets) - 1, acc, T,
          _host_scalars(alpha), _ptr(_dev(global_key)), _ptr(d1p), _ptr(tl), _ptr(ml), _stream())
     return d1p, tl, ml
+
+
+def pfe_verify_proofs_batched(commitments, ct1_affine, responses, challenges, coefficients):
+    """verify_batch (weighted_pfe/src/proof.rs:74-122) given the hashed challenges and the
+    verifier's nonzero coefficients (canonical ints)."""
+    B = len(responses)
+    res = ctypes.c_int(0)
+    call("bte_pfe_verify_proofs_batched", _ptr(_dev(commitments)), _ptr(_dev(ct1_affine)), _host_scalars(responses),
+         _host_scalars(challenges), _host_scalars(coefficients), B, ctypes.byref(res), _stream())
+    return bool(res.value)
 
 
 def pfe_precompute_batched(ct1_affine, total_lines, batch_size):

@@ -28,6 +28,7 @@ int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine* q, size_t
 int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
 int bte_final_exp_hard(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
 int bte_lagrange_at_zero(size_t total_weight, const uint64_t* selected, size_t n, uint8_t* out);
+int bte_g1_msm(const bte_p1_affine* points, const uint8_t* scalars, size_t n, bte_p1* out, void* stream);
 int bte_fft_g1_batched(bte_p1* values, size_t n, size_t count, int inverse, int scaled, void* stream);
 int bte_fft_cyclotomic_batched(bte_fp12* values, size_t n, size_t count, int inverse, void* stream);
 int bte_miller_loop_prepared_tiled(const bte_fp6* lines, size_t ntables, const bte_p1_affine* p, size_t n,
@@ -494,6 +495,71 @@ extern "C" int bte_pfe_open_prepared_batched(const bte_p1* shares, size_t n_acce
     BTE_HIP(hipStreamSynchronize(s));
     pt.publish();
   }
+  return BTE_OK;
+}
+
+__global__ void k_pfe_proof_bases(const g1a* __restrict__ commitments, const g1a* __restrict__ ct1, size_t B,
+                                  g1a* __restrict__ bases) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (g > 2 * B) return;
+  if (g == 0) {
+    g1a G;
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+      G.x.l[j] = G1_GEN[0][j];
+      G.y.l[j] = G1_GEN[1][j];
+    }
+    bases[0] = G;
+  } else {
+    bases[g] = g <= B ? commitments[g - 1] : ct1[g - 1 - B];
+  }
+}
+
+// weighted_pfe/src/proof.rs:74-122 verify_batch after the transcript hashing: the Schnorr
+// equations s_i G = A_i + e_i C_i folded with nonzero coefficients c_i into one G1 MSM of 2B+1
+// terms, sum c_i s_i G - sum c_i A_i - sum c_i e_i C_i == identity.  responses s_i, challenges
+// e_i and coefficients c_i are canonical scalars (HOST); commitments A_i and ct1 C_i device.
+extern "C" int bte_pfe_verify_proofs_batched(const bte_p1_affine* commitments, const bte_p1_affine* ct1,
+                                             const uint8_t* responses, const uint8_t* challenges,
+                                             const uint8_t* coefficients, size_t batch_size, int* result,
+                                             void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t B = batch_size;
+  if (!result) return set_error(BTE_EINVAL, "pfe_verify_proofs: null result");
+  *result = 0;
+  if (B == 0) return BTE_OK;  // the reference rejects an empty batch
+  if (!commitments || !ct1 || !responses || !challenges || !coefficients)
+    return set_error(BTE_EINVAL, "pfe_verify_proofs: null pointer");
+  std::vector<bteh::Fr> sc(2 * B + 1);
+  bteh::Fr gsum = bteh::fr_from_u64(0), zero = bteh::fr_from_u64(0);
+  for (size_t i = 0; i < B; ++i) {
+    bteh::Fr c, e, r;
+    memcpy(c.l, coefficients + 32 * i, 32);
+    memcpy(e.l, challenges + 32 * i, 32);
+    memcpy(r.l, responses + 32 * i, 32);
+    gsum = bteh::fr_add(gsum, bteh::fr_mul(c, r));
+    sc[1 + i] = bteh::fr_sub(zero, c);
+    sc[1 + B + i] = bteh::fr_sub(zero, bteh::fr_mul(c, e));
+  }
+  sc[0] = gsum;
+  PScratch ps(s);
+  g1a* bases;
+  uint8_t* d_sc;
+  g1j* out;
+  PFE_TRY(ps.get(&bases, 2 * B + 1));
+  PFE_TRY(ps.upload(&d_sc, (const uint8_t*)sc.data(), 32 * (2 * B + 1)));
+  PFE_TRY(ps.get(&out, 1));
+  PFE_TRY(ps.done());
+  hipLaunchKernelGGL(k_pfe_proof_bases, dim3(blocks_for(2 * B + 1, 256)), dim3(256), 0, s, (const g1a*)commitments,
+                     (const g1a*)ct1, B, bases);
+  PFE_TRY(bte_g1_msm((const bte_p1_affine*)bases, d_sc, 2 * B + 1, (bte_p1*)out, s));
+  g1j h;
+  BTE_HIP(hipMemcpyAsync(&h, out, sizeof(g1j), hipMemcpyDeviceToHost, s));
+  BTE_HIP(hipStreamSynchronize(s));
+  bool zero_z = true;
+  for (int j = 0; j < 12; ++j) zero_z = zero_z && h.z.l[j] == 0;
+  *result = zero_z ? 1 : 0;
+  BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
 
