@@ -6,10 +6,12 @@
 // 32-byte little-endian scalar; an empty input gives the identity.
 //
 // Pipeline (one stream, all device work except the last serial combination):
-//   1. digits    one thread per point: signed c-bit window digits d in [-(2^(c-1)-1), 2^(c-1)],
-//                global histogram of |d| per (window, bucket)
+//   1. digits    signed c-bit window digits d in [-(2^(c-1)-1), 2^(c-1)] and a histogram of |d|
+//                per (window, bucket): for c <= 16 one workgroup per (window, slice of points)
+//                counting in LDS, else one thread per point with global counters
 //   2. scan      exclusive scan of the histogram -> bucket offsets in one flat sorted array
-//   3. scatter   counting-sort placement of (point index, sign) into bucket order
+//   3. scatter   counting-sort placement of (point index, sign) into bucket order (LDS cursors
+//                per (window, slice) for c <= 16)
 //   4. accumulate  the sorted array is cut into chunks of K entries, one thread per chunk; a
 //                thread sums each run of equal bucket with mixed Jacobian+affine additions.  A
 //                bucket wholly inside a chunk is written directly; pieces of buckets that
@@ -194,6 +196,86 @@ __global__ void k_msm_scatter(const int32_t* __restrict__ digits, size_t n, int 
   const uint32_t b = (uint32_t)(w * M) + (uint32_t)(d < 0 ? -d : d) - 1u;
   const uint32_t pos = offsets[b] + atomicAdd(&cursor[b], 1u);
   sorted[pos] = (uint32_t)i | (d < 0 ? 0x80000000u : 0u);
+}
+
+// Counting sort with per-workgroup histograms in LDS (windows of up to 16 bits).  Workgroup
+// (w, s) owns window w of the points in slice s: it counts that slice's digits in LDS and writes
+// the M counts to hist[w][s][.]; a column scan turns them into per-slice starts inside each
+// bucket; the scatter pass recomputes the digits and places entries through LDS cursors.  No
+// global atomics, and the workgroups of one slice read the same scalars (with S a multiple of
+// 8 they sit on one XCD and share its L2).
+BTE_DEV int scalar_window_digit(const uint32_t* __restrict__ sc, int c, int w, uint32_t M) {
+  uint32_t s[9];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s[k] = sc[k];
+  s[7] &= 0x7fffffffu;  // nbits = 255
+  s[8] = 0;
+  const uint32_t mask = (1u << c) - 1u;
+  int carry = 0, d = 0;
+  for (int v = 0; v <= w; ++v) {  // the recoding of k_msm_digits up to window w
+    const int bit = v * c;
+    const int word = bit >> 5, off = bit & 31;
+    const uint64_t two = (uint64_t)s[word] | ((uint64_t)s[word + 1] << 32);
+    d = (int)((two >> off) & mask) + carry;
+    carry = d > (int)M;
+    if (carry) d -= (int)(1u << c);
+  }
+  return d;
+}
+
+template <int LDS_WORDS, int T>
+__global__ void __launch_bounds__(T) k_msm_hist(const uint32_t* __restrict__ scalars, size_t n, int c, uint32_t M,
+                                                uint32_t S, size_t P, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[LDS_WORDS];
+  const uint32_t w = blockIdx.x / S, s = blockIdx.x - w * S;
+  for (uint32_t b = threadIdx.x; b < M; b += T) h[b] = 0;
+  __syncthreads();
+  const size_t lo = (size_t)s * P, hi = lo + P < n ? lo + P : n;
+  for (size_t i = lo + threadIdx.x; i < hi; i += T) {
+    const int d = scalar_window_digit(scalars + 8 * i, c, (int)w, M);
+    if (d) atomicAdd(&h[(uint32_t)(d < 0 ? -d : d) - 1u], 1u);
+  }
+  __syncthreads();
+  uint32_t* out = hist + ((size_t)w * S + s) * M;
+  for (uint32_t b = threadIdx.x; b < M; b += T) out[b] = h[b];
+}
+
+// hist[w][s][b] <- sum_{s' < s} hist[w][s'][b]; counts[w M + b] <- the column total.
+__global__ void k_msm_hist_colscan(uint32_t* __restrict__ hist, int nw, uint32_t M, uint32_t S,
+                                   uint32_t* __restrict__ counts) {
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= (size_t)nw * M) return;
+  const size_t w = t / M, b = t - w * M;
+  uint32_t* col = hist + w * S * M + b;
+  uint32_t run = 0;
+  for (uint32_t s = 0; s < S; ++s) {
+    const uint32_t v = col[(size_t)s * M];
+    col[(size_t)s * M] = run;
+    run += v;
+  }
+  counts[t] = run;
+}
+
+template <int LDS_WORDS, int T>
+__global__ void __launch_bounds__(T) k_msm_scatter_slices(const uint32_t* __restrict__ scalars, size_t n, int c,
+                                                          uint32_t M, uint32_t S, size_t P,
+                                                          const uint32_t* __restrict__ hist,
+                                                          const uint32_t* __restrict__ offsets,
+                                                          uint32_t* __restrict__ sorted) {
+  __shared__ uint32_t cur[LDS_WORDS];
+  const uint32_t w = blockIdx.x / S, s = blockIdx.x - w * S;
+  const uint32_t* pre = hist + ((size_t)w * S + s) * M;
+  const uint32_t* off = offsets + (size_t)w * M;
+  for (uint32_t b = threadIdx.x; b < M; b += T) cur[b] = off[b] + pre[b];
+  __syncthreads();
+  const size_t lo = (size_t)s * P, hi = lo + P < n ? lo + P : n;
+  for (size_t i = lo + threadIdx.x; i < hi; i += T) {
+    const int d = scalar_window_digit(scalars + 8 * i, c, (int)w, M);
+    if (d) {
+      const uint32_t pos = atomicAdd(&cur[(uint32_t)(d < 0 ? -d : d) - 1u], 1u);
+      sorted[pos] = (uint32_t)i | (d < 0 ? 0x80000000u : 0u);
+    }
+  }
 }
 
 template <class F>
@@ -494,6 +576,38 @@ struct Carver {
   }
 };
 
+// Slices per window for the LDS-histogram sort: about 512 workgroups in all, slices of at least
+// 2048 points, S a multiple of 8 from 8 up, and at most 2^23 histogram counters.
+static uint32_t sort_slices(size_t n, int nw, uint32_t M) {
+  size_t S = std::max<size_t>(1, std::min<size_t>(512 / nw, n / 2048));
+  if (S >= 8) S &= ~(size_t)7;
+  while (S > 1 && (size_t)nw * S * M > ((size_t)1 << 23)) S = S > 8 ? S - 8 : S - 1;
+  return (uint32_t)S;
+}
+
+// LDS tier by bucket count: counters per workgroup and threads per workgroup.
+template <int W, int T>
+static void lds_hist_launch(bool scatter, unsigned nwg, const uint32_t* scalars, size_t n, int c, uint32_t M,
+                            uint32_t S, size_t P, uint32_t* hist, const uint32_t* offsets, uint32_t* sorted,
+                            hipStream_t s) {
+  if (!scatter)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_hist<W, T>), dim3(nwg), dim3(T), 0, s, scalars, n, c, M, S, P, hist);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_scatter_slices<W, T>), dim3(nwg), dim3(T), 0, s, scalars, n, c, M, S,
+                       P, (const uint32_t*)hist, offsets, sorted);
+}
+
+static void lds_sort_pass(bool scatter, unsigned nwg, const uint32_t* scalars, size_t n, int c, uint32_t M,
+                          uint32_t S, size_t P, uint32_t* hist, const uint32_t* offsets, uint32_t* sorted,
+                          hipStream_t s) {
+  if (M <= 2048)
+    lds_hist_launch<2048, 256>(scatter, nwg, scalars, n, c, M, S, P, hist, offsets, sorted, s);
+  else if (M <= 8192)
+    lds_hist_launch<8192, 512>(scatter, nwg, scalars, n, c, M, S, P, hist, offsets, sorted, s);
+  else
+    lds_hist_launch<32768, 1024>(scatter, nwg, scalars, n, c, M, S, P, hist, offsets, sorted, s);
+}
+
 template <class F>
 static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint32_t* index,
                    const uint8_t* scalars, size_t n, jac<F>* out, hipStream_t s) {
@@ -516,20 +630,28 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
   const int ntiles = (int)((nb + 1023) / 1024);
   if (ntiles > (1 << 20)) return set_error(BTE_EINVAL, "msm: window too wide");
 
+  // Windows of up to 16 bits sort through LDS histograms; wider ones through global counters.
+  const bool lds_sort = sh.M <= 32768;
+  const uint32_t S_sort = lds_sort ? sort_slices(n, sh.nw, sh.M) : 0;
+
   Workspace* ws = workspace_for(s);
   std::lock_guard<std::mutex> ws_lock(ws->mu);
   Carver cv(s);
-  int32_t* digits;
-  uint32_t *counts, *offsets, *cursor, *sorted, *tile_sums;
+  int32_t* digits = nullptr;
+  uint32_t *counts, *offsets, *cursor = nullptr, *sorted, *tile_sums, *hist = nullptr;
   jac<F> *buckets, *heads, *tails, *seg_acc, *seg_run, *wsum;
   int* err;
   uint32_t *big, *nbig;
   const size_t max_big = nchunks / FIX_SERIAL + 1;
   int rc;
-  cv.add(&digits, E_max);
+  if (lds_sort) {
+    cv.add(&hist, (size_t)sh.nw * S_sort * sh.M);
+  } else {
+    cv.add(&digits, E_max);
+    cv.add(&cursor, nb);
+  }
   cv.add(&counts, nb);
   cv.add(&offsets, (size_t)nb + 1);
-  cv.add(&cursor, nb);
   cv.add(&sorted, E_max);
   cv.add(&tile_sums, ntiles);
   cv.add(&buckets, nb);
@@ -545,19 +667,32 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
 
   PhaseTimer pt(s);
   pt.begin();
-  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, counts, (size_t)nb, 0u);
-  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, cursor, (size_t)nb, 0u);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)err, (size_t)1, 0u);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nbig, (size_t)1, 0u);
-  hipLaunchKernelGGL(k_msm_digits, dim3(blocks_for(n, 256)), dim3(256), 0, s,
-                     (const uint32_t*)scalars, n, sh.c, sh.nw, sh.M, digits, counts);
+  const size_t P_sort = lds_sort ? (n + S_sort - 1) / S_sort : 0;
+  const unsigned nwg_sort = (unsigned)(sh.nw * S_sort);
+  if (lds_sort) {
+    lds_sort_pass(false, nwg_sort, (const uint32_t*)scalars, n, sh.c, sh.M, S_sort, P_sort, hist, nullptr, nullptr,
+                  s);
+    hipLaunchKernelGGL(k_msm_hist_colscan, dim3(blocks_for(nb, 256)), dim3(256), 0, s, hist, sh.nw, sh.M, S_sort,
+                       counts);
+  } else {
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, counts, (size_t)nb, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, cursor, (size_t)nb, 0u);
+    hipLaunchKernelGGL(k_msm_digits, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                       (const uint32_t*)scalars, n, sh.c, sh.nw, sh.M, digits, counts);
+  }
   pt.mark("digits");
   hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(256), 0, s, counts, (size_t)nb, offsets, tile_sums);
   hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(256), 0, s, tile_sums, ntiles, offsets + nb);
   hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(nb, 256)), dim3(256), 0, s, offsets, (size_t)nb,
                      (const uint32_t*)tile_sums);
-  hipLaunchKernelGGL(k_msm_scatter, dim3(blocks_for(E_max, 256)), dim3(256), 0, s,
-                     (const int32_t*)digits, n, sh.nw, sh.M, (const uint32_t*)offsets, cursor, sorted);
+  if (lds_sort)
+    lds_sort_pass(true, nwg_sort, (const uint32_t*)scalars, n, sh.c, sh.M, S_sort, P_sort, hist,
+                  (const uint32_t*)offsets, sorted, s);
+  else
+    hipLaunchKernelGGL(k_msm_scatter, dim3(blocks_for(E_max, 256)), dim3(256), 0, s,
+                       (const int32_t*)digits, n, sh.nw, sh.M, (const uint32_t*)offsets, cursor, sorted);
   pt.mark("sort");
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_accumulate<F>), dim3(blocks_for(nchunks, 64)), dim3(64), 0, s,
                      points, index, n_points, (const uint32_t*)sorted, (const uint32_t*)offsets, nb,
