@@ -18,8 +18,9 @@
 //                cross chunk boundaries go to per-chunk head/tail slots.  Chunking keeps the
 //                work per thread bounded however skewed the buckets are (sparse scalars pile
 //                onto power-of-two buckets).
-//   5. fixup     one thread per bucket that crosses a few chunks adds its pieces; buckets that
-//                cross many (sparse-scalar pile-ups) get a 64-thread workgroup and a tree
+//   5. fixup     a bucket split over two chunks of one workgroup is closed by the accumulation
+//                lane holding its tail; other split buckets are queued: one thread each when they
+//                cross a few chunks, a 64-thread workgroup and a tree for the sparse-scalar pile-ups
 //   6. segsum    per window, segments of L buckets: running sums give
 //                acc_j = sum_k (k+1) B_{jL+k} and run_j = sum_k B_{jL+k}
 //   7. window sums  per window one workgroup tree-sums A = sum_j acc_j and, for each bit b of
@@ -38,18 +39,31 @@
 
 namespace bte {
 
-static constexpr int MSM_CHUNK = 32;  // sorted entries per accumulation thread
+// Sorted entries per accumulation thread: K = max(MSM_CHUNK_MIN, ceil(E / tstar)), where tstar,
+// the accumulation thread count, is set on the host to a whole number of full-occupancy rounds
+// of the device so the last round is not left part-empty.  Kernels that need K recompute it from
+// the entry count E = offsets[nb].
+static constexpr uint32_t MSM_CHUNK_MIN = 16;
+static constexpr int ACC_ROUNDS = 2;
+BTE_DEV uint32_t chunk_size(uint32_t E, uint32_t tstar) {
+  const uint32_t k = (uint32_t)(((uint64_t)E + tstar - 1) / tstar);
+  return k > MSM_CHUNK_MIN ? k : MSM_CHUNK_MIN;
+}
 
 template <class F> struct MsmTraits;
-// Threads per window-sum workgroup: as many as the LDS tree fits (147 KB of Jacobian points),
-// so each (window, sum) has a short serial chain.
+// Threads per window-sum workgroup: 512 keeps a workgroup to two wavefronts per SIMD, so the
+// full Jacobian addition fits its registers without spilling (1024 spilled and was slower).
+// ACC_WAVES: minimum wavefronts per SIMD asked of the accumulation kernel (G1 fits three with a
+// few bytes of spill in its closing addition; G2 needs the whole register file).
 template <> struct MsmTraits<fp> {
   using HF = bteh::Fp;
-  static constexpr int REDUCE_THREADS = 1024;
+  static constexpr int REDUCE_THREADS = 512;
+  static constexpr int ACC_WAVES = 3;
 };
 template <> struct MsmTraits<fp2> {
   using HF = bteh::Fp2;
   static constexpr int REDUCE_THREADS = 512;
+  static constexpr int ACC_WAVES = 1;
 };
 
 struct MsmShape {
@@ -278,97 +292,144 @@ __global__ void __launch_bounds__(T) k_msm_scatter_slices(const uint32_t* __rest
   }
 }
 
+// Accumulation workgroups are one wavefront: chunk t is lane t % 64 of workgroup t / 64.
+static constexpr int ACC_BLOCK = 64;
+
 template <class F>
-__global__ void __launch_bounds__(64) k_msm_accumulate(const aff<F>* __restrict__ points, const uint32_t* __restrict__ index,
-                                 size_t n_points, const uint32_t* __restrict__ sorted,
-                                 const uint32_t* __restrict__ offsets, uint32_t nb,
-                                 jac<F>* __restrict__ buckets, jac<F>* __restrict__ heads,
-                                 jac<F>* __restrict__ tails, int* __restrict__ err) {
+__global__ void __launch_bounds__(ACC_BLOCK, MsmTraits<F>::ACC_WAVES)
+    k_msm_accumulate(const aff<F>* __restrict__ points, const uint32_t* __restrict__ index, size_t n_points,
+                     const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ offsets, uint32_t nb,
+                     uint32_t tstar, jac<F>* __restrict__ buckets, jac<F>* __restrict__ heads,
+                     jac<F>* __restrict__ tails, int* __restrict__ err) {
   const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   const uint32_t E = offsets[nb];
-  const size_t start = t * MSM_CHUNK;
-  if (start >= E) return;
-  const size_t stop = start + MSM_CHUNK < E ? start + MSM_CHUNK : E;
-  // Bucket holding `start`: offsets[lo] <= start < offsets[hi], narrowed to hi = lo + 1.
-  uint32_t lo = 0, hi = nb;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (offsets[mid] <= start) lo = mid; else hi = mid;
-  }
-  uint32_t b = lo, bs = offsets[b], be = offsets[b + 1];
+  const uint32_t K = chunk_size(E, tstar);
+  const size_t start = t * K;
+  const size_t stop = start + K < E ? start + K : E;
   jac<F> acc = jac_identity<F>();
-  auto emit = [&]() {
-    if (bs < start) heads[t] = acc;
-    else if (be > stop) tails[t] = acc;
-    else buckets[b] = acc;
-  };
-  for (size_t pos = start; pos < stop; ++pos) {
-    if (pos == be) {
-      emit();
-      acc = jac_identity<F>();
-      do {
-        ++b;
-        bs = be;
-        be = offsets[b + 1];
-      } while (be == bs);
+  uint32_t b = 0;
+  bool pending = false;  // last piece is a tail that the next lane's head completes
+  if (start < E) {
+    // Bucket holding `start`: offsets[lo] <= start < offsets[hi], narrowed to hi = lo + 1.
+    uint32_t lo = 0, hi = nb;
+    while (hi - lo > 1) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (offsets[mid] <= start) lo = mid; else hi = mid;
     }
-    const uint32_t e = sorted[pos];
-    uint32_t idx = e & 0x7fffffffu;
-    if (index) {
-      idx = index[idx];
-      if (idx >= n_points) {
-        *err = 1;
-        continue;
+    b = lo;
+    uint32_t bs = offsets[b], be = offsets[b + 1];
+    auto emit = [&]() {
+      if (bs < start) heads[t] = acc;
+      else if (be > stop) tails[t] = acc;
+      else buckets[b] = acc;
+    };
+    for (size_t pos = start; pos < stop; ++pos) {
+      if (pos == be) {
+        emit();
+        acc = jac_identity<F>();
+        do {
+          ++b;
+          bs = be;
+          be = offsets[b + 1];
+        } while (be == bs);
       }
+      const uint32_t e = sorted[pos];
+      uint32_t idx = e & 0x7fffffffu;
+      if (index) {
+        idx = index[idx];
+        if (idx >= n_points) {
+          *err = 1;
+          continue;
+        }
+      }
+      aff<F> q = points[idx];
+      if (e >> 31) q = aff_neg(q);
+      acc = jac_add_aff(acc, q);
     }
-    aff<F> q = points[idx];
-    if (e >> 31) q = aff_neg(q);
-    acc = jac_add_aff(acc, q);
+    // A bucket that starts in this chunk and ends in the next one, held by the next lane of
+    // this workgroup, is closed here; k_msm_fixup leaves exactly these buckets alone.
+    pending = bs >= start && be > stop && (be - 1) / K == t + 1 && threadIdx.x + 1 < ACC_BLOCK;
+    if (!pending) emit();
   }
-  emit();
+  __syncthreads();  // the workgroup's head pieces are written
+  if (pending) buckets[b] = jac_add(acc, heads[t + 1]);
 }
 
-// Buckets spanning more than FIX_SERIAL chunks (the sparse-scalar pile-ups) are queued for
-// k_msm_fixup_big instead of being summed by one thread.
+// Appends the lanes with `want` set to a device queue with one atomic per wavefront; every lane
+// of the wavefront must call it.
+BTE_DEV uint32_t wave_append(uint32_t* counter, bool want) {
+  const uint64_t mask = __ballot(want);
+  if (!mask) return 0;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  const int lane = __lane_id();
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// Buckets that cross chunk boundaries and were not closed by k_msm_accumulate: those spanning at
+// most FIX_SERIAL + 1 chunks go to the `mid` queue (one thread each), the rest (sparse-scalar
+// pile-ups) to the `big` queue (one workgroup each).  Empty buckets are set to the identity.
 static constexpr uint32_t FIX_SERIAL = 8;
 
 template <class F>
-__global__ void __launch_bounds__(64) k_msm_fixup(const uint32_t* __restrict__ offsets, uint32_t nb,
-                                                  jac<F>* __restrict__ buckets, const jac<F>* __restrict__ heads,
-                                                  const jac<F>* __restrict__ tails, uint32_t* __restrict__ big,
-                                                  uint32_t* __restrict__ nbig) {
+__global__ void __launch_bounds__(64) k_msm_fixup_queue(const uint32_t* __restrict__ offsets, uint32_t nb,
+                                                        uint32_t tstar, jac<F>* __restrict__ buckets,
+                                                        uint32_t* __restrict__ mid, uint32_t* __restrict__ nmid,
+                                                        uint32_t* __restrict__ big, uint32_t* __restrict__ nbig) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  const uint32_t bs = offsets[b], be = offsets[b + 1];
-  if (bs == be) {
-    buckets[b] = jac_identity<F>();
-    return;
+  const uint32_t K = chunk_size(offsets[nb], tstar);
+  bool is_mid = false, is_big = false;
+  if (b < nb) {
+    const uint32_t bs = offsets[b], be = offsets[b + 1];
+    if (bs == be) {
+      buckets[b] = jac_identity<F>();
+    } else {
+      const uint32_t cs = bs / K, ce = (be - 1) / K;
+      const bool closed = ce == cs || (ce == cs + 1 && cs / ACC_BLOCK == ce / ACC_BLOCK);
+      is_big = ce - cs > FIX_SERIAL;
+      is_mid = !closed && !is_big;
+    }
   }
-  const uint32_t cs = bs / MSM_CHUNK, ce = (be - 1) / MSM_CHUNK;
-  if (cs == ce) return;
-  if (ce - cs > FIX_SERIAL) {
-    big[atomicAdd(nbig, 1u)] = b;
-    return;
+  const uint32_t qm = wave_append(nmid, is_mid);
+  const uint32_t qb = wave_append(nbig, is_big);
+  if (is_mid) mid[qm] = b;
+  if (is_big) big[qb] = b;
+}
+
+template <class F>
+__global__ void __launch_bounds__(64) k_msm_fixup_mid(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t tstar,
+                                                      jac<F>* __restrict__ buckets, const jac<F>* __restrict__ heads,
+                                                      const jac<F>* __restrict__ tails, const uint32_t* __restrict__ mid,
+                                                      const uint32_t* __restrict__ nmid) {
+  const uint32_t nq = *nmid;
+  const uint32_t K = chunk_size(offsets[nb], tstar);
+  for (uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x; qi < nq; qi += gridDim.x * blockDim.x) {
+    const uint32_t b = mid[qi];
+    const uint32_t bs = offsets[b], be = offsets[b + 1];
+    const uint32_t cs = bs / K, ce = (be - 1) / K;
+    jac<F> acc = tails[cs];
+    for (uint32_t c = cs + 1; c <= ce; ++c) acc = jac_add(acc, heads[c]);
+    buckets[b] = acc;
   }
-  jac<F> acc = tails[cs];
-  for (uint32_t c = cs + 1; c <= ce; ++c) acc = jac_add(acc, heads[c]);
-  buckets[b] = acc;
 }
 
 // 64-thread workgroups take the queued buckets in turn (grid-stride over the device-side queue
 // length): strided partial sums, then a tree in LDS.
 template <class F>
-__global__ void __launch_bounds__(64) k_msm_fixup_big(const uint32_t* __restrict__ offsets,
+__global__ void __launch_bounds__(64) k_msm_fixup_big(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t tstar,
                                                       jac<F>* __restrict__ buckets, const jac<F>* __restrict__ heads,
                                                       const jac<F>* __restrict__ tails, const uint32_t* __restrict__ big,
                                                       const uint32_t* __restrict__ nbig) {
   __shared__ jac<F> sh[64];
   const uint32_t nq = *nbig;
+  const uint32_t K = chunk_size(offsets[nb], tstar);
   const int tid = threadIdx.x;
   for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {  // uniform per workgroup
     const uint32_t b = big[qi];
     const uint32_t bs = offsets[b], be = offsets[b + 1];
-    const uint32_t cs = bs / MSM_CHUNK, ce = (be - 1) / MSM_CHUNK;
+    const uint32_t cs = bs / K, ce = (be - 1) / K;
     jac<F> acc = tid == 0 ? tails[cs] : jac_identity<F>();
     for (uint32_t c = cs + 1 + tid; c <= ce; c += 64) acc = jac_add(acc, heads[c]);
     sh[tid] = acc;
@@ -576,6 +637,33 @@ struct Carver {
   }
 };
 
+// Accumulation threads: ACC_ROUNDS full-occupancy rounds of one-wavefront workgroups on this
+// device (queried once per device), and no more than chunks of the minimum size need.
+template <class F>
+static uint32_t acc_threads(size_t E_max) {
+  static std::mutex mu;
+  static int slots[64];
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  int sl;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (!slots[dev]) {
+      int cus = 0, per_cu = 0;
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_msm_accumulate<F>),
+                                                       ACC_BLOCK, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 4;
+      slots[dev] = cus * per_cu;
+    }
+    sl = slots[dev];
+  }
+  const size_t full = (size_t)sl * ACC_BLOCK * ACC_ROUNDS;
+  const size_t need = (E_max + MSM_CHUNK_MIN - 1) / MSM_CHUNK_MIN;
+  return (uint32_t)std::max<size_t>(1, std::min(full, need));
+}
+
 // Slices per window for the LDS-histogram sort: about 512 workgroups in all, slices of at least
 // 2048 points, S a multiple of 8 from 8 up, and at most 2^23 histogram counters.
 static uint32_t sort_slices(size_t n, int nw, uint32_t M) {
@@ -624,7 +712,7 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
   const MsmShape sh = msm_shape(curve, n);
   const uint32_t nb = (uint32_t)sh.nw * sh.M;
   const size_t E_max = n * (size_t)sh.nw;
-  const size_t nchunks = (E_max + MSM_CHUNK - 1) / MSM_CHUNK;
+  const uint32_t tstar = acc_threads<F>(E_max);
   const size_t S = (size_t)1 << sh.logS;
   const int nsum = 1 + sh.logS;
   const int ntiles = (int)((nb + 1023) / 1024);
@@ -641,8 +729,8 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
   uint32_t *counts, *offsets, *cursor = nullptr, *sorted, *tile_sums, *hist = nullptr;
   jac<F> *buckets, *heads, *tails, *seg_acc, *seg_run, *wsum;
   int* err;
-  uint32_t *big, *nbig;
-  const size_t max_big = nchunks / FIX_SERIAL + 1;
+  uint32_t *big, *nbig, *mid, *nmid;
+  const size_t max_big = tstar / FIX_SERIAL + 1;
   int rc;
   if (lds_sort) {
     cv.add(&hist, (size_t)sh.nw * S_sort * sh.M);
@@ -655,20 +743,23 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
   cv.add(&sorted, E_max);
   cv.add(&tile_sums, ntiles);
   cv.add(&buckets, nb);
-  cv.add(&heads, nchunks);
-  cv.add(&tails, nchunks);
+  cv.add(&heads, tstar);
+  cv.add(&tails, tstar);
   cv.add(&seg_acc, sh.nw * S);
   cv.add(&seg_run, sh.nw * S);
   cv.add(&wsum, (size_t)sh.nw * nsum);
   cv.add(&err, 1);
   cv.add(&big, max_big);
   cv.add(&nbig, 1);
+  cv.add(&mid, nb);
+  cv.add(&nmid, 1);
   if ((rc = cv.commit(ws))) return rc;
 
   PhaseTimer pt(s);
   pt.begin();
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)err, (size_t)1, 0u);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nbig, (size_t)1, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nmid, (size_t)1, 0u);
   const size_t P_sort = lds_sort ? (n + S_sort - 1) / S_sort : 0;
   const unsigned nwg_sort = (unsigned)(sh.nw * S_sort);
   if (lds_sort) {
@@ -694,15 +785,17 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
     hipLaunchKernelGGL(k_msm_scatter, dim3(blocks_for(E_max, 256)), dim3(256), 0, s,
                        (const int32_t*)digits, n, sh.nw, sh.M, (const uint32_t*)offsets, cursor, sorted);
   pt.mark("sort");
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_accumulate<F>), dim3(blocks_for(nchunks, 64)), dim3(64), 0, s,
-                     points, index, n_points, (const uint32_t*)sorted, (const uint32_t*)offsets, nb,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_accumulate<F>), dim3(blocks_for(tstar, ACC_BLOCK)), dim3(ACC_BLOCK), 0, s,
+                     points, index, n_points, (const uint32_t*)sorted, (const uint32_t*)offsets, nb, tstar,
                      buckets, heads, tails, err);
   pt.mark("accumulate");
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup<F>), dim3(blocks_for(nb, 64)), dim3(64), 0, s,
-                     (const uint32_t*)offsets, nb, buckets, (const jac<F>*)heads, (const jac<F>*)tails, big,
-                     nbig);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_queue<F>), dim3(blocks_for(nb, 64)), dim3(64), 0, s,
+                     (const uint32_t*)offsets, nb, tstar, buckets, mid, nmid, big, nbig);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_mid<F>), dim3((unsigned)std::min<size_t>(blocks_for(nb, 64), 4096)),
+                     dim3(64), 0, s, (const uint32_t*)offsets, nb, tstar, buckets, (const jac<F>*)heads,
+                     (const jac<F>*)tails, (const uint32_t*)mid, (const uint32_t*)nmid);
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_big<F>), dim3((unsigned)std::min<size_t>(max_big, 2048)), dim3(64), 0, s,
-                     (const uint32_t*)offsets, buckets, (const jac<F>*)heads, (const jac<F>*)tails,
+                     (const uint32_t*)offsets, nb, tstar, buckets, (const jac<F>*)heads, (const jac<F>*)tails,
                      (const uint32_t*)big, (const uint32_t*)nbig);
   pt.mark("fixup");
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_segsum<F>), dim3(blocks_for(sh.nw * S, 64)), dim3(64), 0, s,
@@ -764,14 +857,14 @@ static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t
   if (nb64 >= 0x7fffffffu) return set_error(BTE_EINVAL, "msm_batched: %zu buckets exceed the index range", nb64);
   const uint32_t nb = (uint32_t)nb64;
   const size_t E_max = nterms * (size_t)nw;
-  const size_t nchunks = (E_max + MSM_CHUNK - 1) / MSM_CHUNK;
+  const uint32_t tstar = acc_threads<F>(E_max);
   const int ntiles = (int)((nb + 1023) / 1024);
-  const size_t max_big = nchunks / FIX_SERIAL + 1;
+  const size_t max_big = tstar / FIX_SERIAL + 1;
 
   Workspace* ws = workspace_for(s);
   std::lock_guard<std::mutex> ws_lock(ws->mu);
   Carver cv(s);
-  uint32_t *keys, *counts, *offsets, *cursor, *sorted, *tile_sums, *big, *nbig;
+  uint32_t *keys, *counts, *offsets, *cursor, *sorted, *tile_sums, *big, *nbig, *mid, *nmid;
   jac<F> *buckets, *heads, *tails, *win, *seg_acc, *seg_run;
   int* err;
   // bucket reduction in two levels: runs of L = 2^logL buckets per thread, then per window
@@ -784,14 +877,16 @@ static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t
   cv.add(&sorted, E_max);
   cv.add(&tile_sums, ntiles);
   cv.add(&buckets, nb);
-  cv.add(&heads, nchunks);
-  cv.add(&tails, nchunks);
+  cv.add(&heads, tstar);
+  cv.add(&tails, tstar);
   cv.add(&win, nsw);
   cv.add(&seg_acc, nss);
   cv.add(&seg_run, nss);
   cv.add(&err, 1);
   cv.add(&big, max_big);
   cv.add(&nbig, 1);
+  cv.add(&mid, nb);
+  cv.add(&nmid, 1);
   int rc = cv.commit(ws);
   if (rc) return rc;
 
@@ -799,6 +894,7 @@ static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, cursor, (size_t)nb, 0u);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)err, (size_t)1, 0u);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nbig, (size_t)1, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nmid, (size_t)1, 0u);
   if (nterms)
     hipLaunchKernelGGL(k_msm_digits_seg, dim3(blocks_for(nterms, 256)), dim3(256), 0, s, (const uint32_t*)scalars,
                        scalar_index, nterms, seg_offsets, (uint32_t)nseg, c, nw, M, keys, counts);
@@ -809,14 +905,17 @@ static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t
   if (E_max) {
     hipLaunchKernelGGL(k_msm_scatter_keys, dim3(blocks_for(E_max, 256)), dim3(256), 0, s, (const uint32_t*)keys,
                        nterms, nw, (const uint32_t*)offsets, cursor, sorted);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_accumulate<F>), dim3(blocks_for(nchunks, 64)), dim3(64), 0, s,
-                       points, index, index ? n_points : nterms, (const uint32_t*)sorted, (const uint32_t*)offsets,
-                       nb, buckets, heads, tails, err);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_accumulate<F>), dim3(blocks_for(tstar, ACC_BLOCK)), dim3(ACC_BLOCK), 0,
+                       s, points, index, index ? n_points : nterms, (const uint32_t*)sorted, (const uint32_t*)offsets,
+                       nb, tstar, buckets, heads, tails, err);
   }
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup<F>), dim3(blocks_for(nb, 64)), dim3(64), 0, s,
-                     (const uint32_t*)offsets, nb, buckets, (const jac<F>*)heads, (const jac<F>*)tails, big, nbig);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_queue<F>), dim3(blocks_for(nb, 64)), dim3(64), 0, s,
+                     (const uint32_t*)offsets, nb, tstar, buckets, mid, nmid, big, nbig);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_mid<F>), dim3((unsigned)std::min<size_t>(blocks_for(nb, 64), 4096)),
+                     dim3(64), 0, s, (const uint32_t*)offsets, nb, tstar, buckets, (const jac<F>*)heads,
+                     (const jac<F>*)tails, (const uint32_t*)mid, (const uint32_t*)nmid);
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_big<F>), dim3((unsigned)std::min<size_t>(max_big, 2048)), dim3(64), 0, s,
-                     (const uint32_t*)offsets, buckets, (const jac<F>*)heads, (const jac<F>*)tails,
+                     (const uint32_t*)offsets, nb, tstar, buckets, (const jac<F>*)heads, (const jac<F>*)tails,
                      (const uint32_t*)big, (const uint32_t*)nbig);
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_segsum<F>), dim3(blocks_for(nss, 64)), dim3(64), 0, s,
                      (const jac<F>*)buckets, (int)nsw, M, logL, logS, seg_acc, seg_run);
