@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(ACC_BLOCK, MsmTraits<F>::ACC_WAVES)
       acc = jac_add_aff(acc, q);
     }
     // A bucket that starts in this chunk and ends in the next one, held by the next lane of
-    // this workgroup, is closed here; k_msm_fixup leaves exactly these buckets alone.
+    // this workgroup, is closed here; k_msm_fixup_queue leaves exactly these buckets alone.
     pending = bs >= start && be > stop && (be - 1) / K == t + 1 && threadIdx.x + 1 < ACC_BLOCK;
     if (!pending) emit();
   }
@@ -355,29 +355,41 @@ __global__ void __launch_bounds__(ACC_BLOCK, MsmTraits<F>::ACC_WAVES)
   if (pending) buckets[b] = jac_add(acc, heads[t + 1]);
 }
 
-// Appends the lanes with `want` set to a device queue with one atomic per wavefront; every lane
-// of the wavefront must call it.
-BTE_DEV uint32_t wave_append(uint32_t* counter, bool want) {
+// Appends the threads with `want` set to a device queue with one atomic per workgroup of T
+// threads; every thread of the workgroup must call it.  sh holds T / 64 + 1 words.
+template <int T>
+BTE_DEV uint32_t block_append(uint32_t* counter, bool want, uint32_t* sh) {
+  const int lane = __lane_id(), wv = threadIdx.x >> 6;
   const uint64_t mask = __ballot(want);
-  if (!mask) return 0;
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  const int lane = __lane_id();
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader);
-  return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+  if (lane == 0) sh[wv] = (uint32_t)__popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int i = 0; i < T / 64; ++i) {
+      const uint32_t c = sh[i];
+      sh[i] = tot;
+      tot += c;
+    }
+    sh[T / 64] = tot ? atomicAdd(counter, tot) : 0u;
+  }
+  __syncthreads();
+  const uint32_t r = sh[T / 64] + sh[wv] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+  __syncthreads();  // sh is reused by the next call
+  return r;
 }
 
 // Buckets that cross chunk boundaries and were not closed by k_msm_accumulate: those spanning at
 // most FIX_SERIAL + 1 chunks go to the `mid` queue (one thread each), the rest (sparse-scalar
 // pile-ups) to the `big` queue (one workgroup each).  Empty buckets are set to the identity.
 static constexpr uint32_t FIX_SERIAL = 8;
+static constexpr int QUEUE_BLOCK = 1024;
 
 template <class F>
-__global__ void __launch_bounds__(64) k_msm_fixup_queue(const uint32_t* __restrict__ offsets, uint32_t nb,
-                                                        uint32_t tstar, jac<F>* __restrict__ buckets,
-                                                        uint32_t* __restrict__ mid, uint32_t* __restrict__ nmid,
-                                                        uint32_t* __restrict__ big, uint32_t* __restrict__ nbig) {
+__global__ void __launch_bounds__(QUEUE_BLOCK) k_msm_fixup_queue(const uint32_t* __restrict__ offsets, uint32_t nb,
+                                                                 uint32_t tstar, jac<F>* __restrict__ buckets,
+                                                                 uint32_t* __restrict__ mid, uint32_t* __restrict__ nmid,
+                                                                 uint32_t* __restrict__ big, uint32_t* __restrict__ nbig) {
+  __shared__ uint32_t sh[QUEUE_BLOCK / 64 + 1];
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t K = chunk_size(offsets[nb], tstar);
   bool is_mid = false, is_big = false;
@@ -392,54 +404,52 @@ __global__ void __launch_bounds__(64) k_msm_fixup_queue(const uint32_t* __restri
       is_mid = !closed && !is_big;
     }
   }
-  const uint32_t qm = wave_append(nmid, is_mid);
-  const uint32_t qb = wave_append(nbig, is_big);
+  const uint32_t qm = block_append<QUEUE_BLOCK>(nmid, is_mid, sh);
+  const uint32_t qb = block_append<QUEUE_BLOCK>(nbig, is_big, sh);
   if (is_mid) mid[qm] = b;
   if (is_big) big[qb] = b;
 }
 
+// One launch for both queues so the two kinds overlap: workgroups below gbig take big buckets in
+// turn (64 threads: strided partial sums over the pieces, then a tree in LDS); the others give
+// each mid bucket one thread that adds its few pieces in order.
 template <class F>
-__global__ void __launch_bounds__(64) k_msm_fixup_mid(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t tstar,
-                                                      jac<F>* __restrict__ buckets, const jac<F>* __restrict__ heads,
-                                                      const jac<F>* __restrict__ tails, const uint32_t* __restrict__ mid,
-                                                      const uint32_t* __restrict__ nmid) {
-  const uint32_t nq = *nmid;
-  const uint32_t K = chunk_size(offsets[nb], tstar);
-  for (uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x; qi < nq; qi += gridDim.x * blockDim.x) {
-    const uint32_t b = mid[qi];
-    const uint32_t bs = offsets[b], be = offsets[b + 1];
-    const uint32_t cs = bs / K, ce = (be - 1) / K;
-    jac<F> acc = tails[cs];
-    for (uint32_t c = cs + 1; c <= ce; ++c) acc = jac_add(acc, heads[c]);
-    buckets[b] = acc;
-  }
-}
-
-// 64-thread workgroups take the queued buckets in turn (grid-stride over the device-side queue
-// length): strided partial sums, then a tree in LDS.
-template <class F>
-__global__ void __launch_bounds__(64) k_msm_fixup_big(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t tstar,
-                                                      jac<F>* __restrict__ buckets, const jac<F>* __restrict__ heads,
-                                                      const jac<F>* __restrict__ tails, const uint32_t* __restrict__ big,
-                                                      const uint32_t* __restrict__ nbig) {
+__global__ void __launch_bounds__(64) k_msm_fixup(const uint32_t* __restrict__ offsets, uint32_t nb, uint32_t tstar,
+                                                  jac<F>* __restrict__ buckets, const jac<F>* __restrict__ heads,
+                                                  const jac<F>* __restrict__ tails, const uint32_t* __restrict__ mid,
+                                                  const uint32_t* __restrict__ nmid, const uint32_t* __restrict__ big,
+                                                  const uint32_t* __restrict__ nbig, uint32_t gbig) {
   __shared__ jac<F> sh[64];
-  const uint32_t nq = *nbig;
   const uint32_t K = chunk_size(offsets[nb], tstar);
   const int tid = threadIdx.x;
-  for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {  // uniform per workgroup
-    const uint32_t b = big[qi];
-    const uint32_t bs = offsets[b], be = offsets[b + 1];
-    const uint32_t cs = bs / K, ce = (be - 1) / K;
-    jac<F> acc = tid == 0 ? tails[cs] : jac_identity<F>();
-    for (uint32_t c = cs + 1 + tid; c <= ce; c += 64) acc = jac_add(acc, heads[c]);
-    sh[tid] = acc;
-    __syncthreads();
-    for (int w = 32; w > 0; w >>= 1) {
-      if (tid < w) sh[tid] = jac_add(sh[tid], sh[tid + w]);
+  if (blockIdx.x < gbig) {
+    const uint32_t nq = *nbig;
+    for (uint32_t qi = blockIdx.x; qi < nq; qi += gbig) {  // uniform per workgroup
+      const uint32_t b = big[qi];
+      const uint32_t bs = offsets[b], be = offsets[b + 1];
+      const uint32_t cs = bs / K, ce = (be - 1) / K;
+      jac<F> acc = tid == 0 ? tails[cs] : jac_identity<F>();
+      for (uint32_t c = cs + 1 + tid; c <= ce; c += 64) acc = jac_add(acc, heads[c]);
+      sh[tid] = acc;
+      __syncthreads();
+      for (int w = 32; w > 0; w >>= 1) {
+        if (tid < w) sh[tid] = jac_add(sh[tid], sh[tid + w]);
+        __syncthreads();
+      }
+      if (tid == 0) buckets[b] = sh[0];
       __syncthreads();
     }
-    if (tid == 0) buckets[b] = sh[0];
-    __syncthreads();
+  } else {
+    const uint32_t nq = *nmid;
+    const uint32_t stride = (gridDim.x - gbig) * 64u;
+    for (uint32_t qi = (blockIdx.x - gbig) * 64u + tid; qi < nq; qi += stride) {
+      const uint32_t b = mid[qi];
+      const uint32_t bs = offsets[b], be = offsets[b + 1];
+      const uint32_t cs = bs / K, ce = (be - 1) / K;
+      jac<F> acc = tails[cs];
+      for (uint32_t c = cs + 1; c <= ce; ++c) acc = jac_add(acc, heads[c]);
+      buckets[b] = acc;
+    }
   }
 }
 
@@ -789,14 +799,15 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
                      points, index, n_points, (const uint32_t*)sorted, (const uint32_t*)offsets, nb, tstar,
                      buckets, heads, tails, err);
   pt.mark("accumulate");
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_queue<F>), dim3(blocks_for(nb, 64)), dim3(64), 0, s,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_queue<F>), dim3(blocks_for(nb, QUEUE_BLOCK)), dim3(QUEUE_BLOCK), 0, s,
                      (const uint32_t*)offsets, nb, tstar, buckets, mid, nmid, big, nbig);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_mid<F>), dim3((unsigned)std::min<size_t>(blocks_for(nb, 64), 4096)),
-                     dim3(64), 0, s, (const uint32_t*)offsets, nb, tstar, buckets, (const jac<F>*)heads,
-                     (const jac<F>*)tails, (const uint32_t*)mid, (const uint32_t*)nmid);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_big<F>), dim3((unsigned)std::min<size_t>(max_big, 2048)), dim3(64), 0, s,
-                     (const uint32_t*)offsets, nb, tstar, buckets, (const jac<F>*)heads, (const jac<F>*)tails,
-                     (const uint32_t*)big, (const uint32_t*)nbig);
+  {
+    const unsigned gbig = (unsigned)std::min<size_t>(max_big, 2048);
+    const unsigned gmid = (unsigned)std::min<size_t>(blocks_for(nb, 64), 4096);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup<F>), dim3(gbig + gmid), dim3(64), 0, s, (const uint32_t*)offsets,
+                       nb, tstar, buckets, (const jac<F>*)heads, (const jac<F>*)tails, (const uint32_t*)mid,
+                       (const uint32_t*)nmid, (const uint32_t*)big, (const uint32_t*)nbig, gbig);
+  }
   pt.mark("fixup");
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_segsum<F>), dim3(blocks_for(sh.nw * S, 64)), dim3(64), 0, s,
                      (const jac<F>*)buckets, sh.nw, sh.M, sh.logL, sh.logS, seg_acc, seg_run);
@@ -909,14 +920,15 @@ static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t
                        s, points, index, index ? n_points : nterms, (const uint32_t*)sorted, (const uint32_t*)offsets,
                        nb, tstar, buckets, heads, tails, err);
   }
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_queue<F>), dim3(blocks_for(nb, 64)), dim3(64), 0, s,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_queue<F>), dim3(blocks_for(nb, QUEUE_BLOCK)), dim3(QUEUE_BLOCK), 0, s,
                      (const uint32_t*)offsets, nb, tstar, buckets, mid, nmid, big, nbig);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_mid<F>), dim3((unsigned)std::min<size_t>(blocks_for(nb, 64), 4096)),
-                     dim3(64), 0, s, (const uint32_t*)offsets, nb, tstar, buckets, (const jac<F>*)heads,
-                     (const jac<F>*)tails, (const uint32_t*)mid, (const uint32_t*)nmid);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_big<F>), dim3((unsigned)std::min<size_t>(max_big, 2048)), dim3(64), 0, s,
-                     (const uint32_t*)offsets, nb, tstar, buckets, (const jac<F>*)heads, (const jac<F>*)tails,
-                     (const uint32_t*)big, (const uint32_t*)nbig);
+  {
+    const unsigned gbig = (unsigned)std::min<size_t>(max_big, 2048);
+    const unsigned gmid = (unsigned)std::min<size_t>(blocks_for(nb, 64), 4096);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup<F>), dim3(gbig + gmid), dim3(64), 0, s, (const uint32_t*)offsets,
+                       nb, tstar, buckets, (const jac<F>*)heads, (const jac<F>*)tails, (const uint32_t*)mid,
+                       (const uint32_t*)nmid, (const uint32_t*)big, (const uint32_t*)nbig, gbig);
+  }
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_segsum<F>), dim3(blocks_for(nss, 64)), dim3(64), 0, s,
                      (const jac<F>*)buckets, (int)nsw, M, logL, logS, seg_acc, seg_run);
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_seg_window<F>), dim3(blocks_for(nsw, 64)), dim3(64), 0, s,
