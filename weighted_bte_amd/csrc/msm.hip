@@ -23,11 +23,12 @@
 //                cross a few chunks, a 64-thread workgroup and a tree for the sparse-scalar pile-ups
 //   6. segsum    per window, segments of L buckets: running sums give
 //                acc_j = sum_k (k+1) B_{jL+k} and run_j = sum_k B_{jL+k}
-//   7. window sums  per window one workgroup tree-sums A = sum_j acc_j and, for each bit b of
-//                the segment index, D_b = sum_{j: bit b of j} run_j, so that
+//   7. window sums  per window, workgroups tree-sum A = sum_j acc_j (in two halves) and, for
+//                each bit b of the segment index, D_b = sum_{j: bit b of j} run_j, so that
 //                T_w = A + L * sum_b 2^b D_b
 //   8. host      sum_w 2^(c w) T_w by one Horner pass over at most 256 doublings; this serial
-//                tail is ~0.1 ms on a host core and would cost milliseconds in one GPU lane.
+//                tail takes a few tenths of a millisecond on a host core and would cost
+//                milliseconds in one GPU lane.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -95,7 +96,7 @@ static MsmShape msm_shape(int curve, size_t n) {
   s.c = c;
   s.nw = (256 + c - 1) / c;
   s.M = 1u << (c - 1);
-  s.logL = std::min(c - 1, 3);
+  s.logL = std::min(c - 1, 2);  // 4 buckets per segment thread: two wavefronts per SIMD in segsum
   s.logS = (c - 1) - s.logL;
   return s;
 }
@@ -471,19 +472,30 @@ __global__ void __launch_bounds__(64) k_msm_segsum(const jac<F>* __restrict__ bu
   seg_run[t] = run;
 }
 
+// Per window w, 2 + logS sums of at most S/2 reduction segments each, one workgroup per sum and
+// every thread busy in the strided phase: sum 0 = sum_{j < ceil(S/2)} acc_j, sum 1 = the other
+// acc_j, sum 2 + b = D_b = sum_{j with bit b set} run_j; T_w = sum 0 + sum 1 + L sum_b 2^b D_b.
 template <class F, int T>
 __global__ void __launch_bounds__(T) k_msm_window_sums(const jac<F>* __restrict__ seg_acc,
                                                        const jac<F>* __restrict__ seg_run, int logS,
                                                        jac<F>* __restrict__ out) {
   __shared__ jac<F> sh[T];
-  const int nsum = 1 + logS;
+  const int nsum = 2 + logS;
   const int w = blockIdx.x / nsum, sid = blockIdx.x % nsum;
   const size_t S = (size_t)1 << logS;
   const int tid = threadIdx.x;
   jac<F> acc = jac_identity<F>();
-  for (size_t j = tid; j < S; j += T) {
-    if (sid == 0) acc = jac_add(acc, seg_acc[w * S + j]);
-    else if ((j >> (sid - 1)) & 1) acc = jac_add(acc, seg_run[w * S + j]);
+  if (sid < 2) {
+    const size_t n0 = (S + 1) / 2;
+    const size_t lo = sid ? n0 : 0, hi = sid ? S : n0;
+    for (size_t j = lo + tid; j < hi; j += T) acc = jac_add(acc, seg_acc[w * S + j]);
+  } else {
+    const int b = sid - 2;
+    const size_t low = ((size_t)1 << b) - 1;
+    for (size_t i = tid; i < S / 2; i += T) {
+      const size_t j = ((i >> b) << (b + 1)) | ((size_t)1 << b) | (i & low);
+      acc = jac_add(acc, seg_run[w * S + j]);
+    }
   }
   sh[tid] = acc;
   __syncthreads();
@@ -724,7 +736,7 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
   const size_t E_max = n * (size_t)sh.nw;
   const uint32_t tstar = acc_threads<F>(E_max);
   const size_t S = (size_t)1 << sh.logS;
-  const int nsum = 1 + sh.logS;
+  const int nsum = 2 + sh.logS;
   const int ntiles = (int)((nb + 1023) / 1024);
   if (ntiles > (1 << 20)) return set_error(BTE_EINVAL, "msm: window too wide");
 
@@ -831,7 +843,8 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
   std::vector<std::vector<int>> at(top + 1);
   for (int w = 0; w < sh.nw; ++w) {
     at[sh.c * w].push_back(w * nsum);
-    for (int b = 0; b < sh.logS; ++b) at[sh.c * w + sh.logL + b].push_back(w * nsum + 1 + b);
+    at[sh.c * w].push_back(w * nsum + 1);
+    for (int b = 0; b < sh.logS; ++b) at[sh.c * w + sh.logL + b].push_back(w * nsum + 2 + b);
   }
   bteh::Jac<HF> acc = bteh::jac_identity<HF>();
   for (int e = top; e >= 0; --e) {
