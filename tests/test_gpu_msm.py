@@ -176,6 +176,27 @@ def test_g1_msm_window_override_same_result(api, c):
     assert g1_jac_affine(api.to_host(out)) == g1_mul_gen(sum(k * s for k, s in zip(ks, sc)))
 
 
+@pytest.mark.parametrize("c,n", [(12, 5000), (16, 20001), (17, 3000)])
+def test_g1_msm_sort_paths(api, c, n):
+    """The LDS-histogram sort at its 8192-counter tier (c = 12) and its 32768-counter tier with
+    several point slices and a short last slice (c = 16), and the global-counter sort of windows
+    wider than 16 bits (c = 17); each with uniform scalars, identical scalars (one bucket per
+    window spanning every accumulation chunk) and one-bit scalars (pile-ups on power-of-two
+    buckets).  Closed form (sum k_i s_i) G, compared as compressed bytes."""
+    rng = random.Random(900 + c)
+    ks = [rng.randrange(1, O.R) for _ in range(n)]
+    bases = api.g1_generator_mul(api.to_device(pack_scalars(ks)))
+    api.set_msm_window(1, n, c)
+    try:
+        for sc in ([rng.randrange(O.R) for _ in range(n)], [0xDEADBEEF] * n,
+                   [1 << rng.randrange(255) for _ in range(n)]):
+            out = api.g1_multi_exp_affine_bytes(bases, api.to_device(pack_scalars(sc)))
+            want = g1_mul_gen(sum(k * s for k, s in zip(ks, sc)) % O.R)
+            assert O.g1_compress(g1_jac_affine(api.to_host(out))) == O.g1_compress(want)
+    finally:
+        api.set_msm_window(1, n, 0)
+
+
 @pytest.mark.parametrize("group", [1, 2])
 def test_batched_msm_uneven_segments(api, group):
     """Many MSMs of uneven sizes (0, 1, 5, 37, 200, 3 terms) in one launch sequence, each equal
