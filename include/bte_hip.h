@@ -291,6 +291,13 @@ int bte_btx_verify_shares_batched(const bte_p1* shares, const uint32_t* party_in
                                   size_t max_batch, const bte_p1_affine* ct1, const uint8_t* valid,
                                   size_t batch_size, int* result, void* stream);
 
+/* weighted_btx/src/proof.rs:56-70 SchnorrProof::verify for B ciphertexts at once, after the host's
+ * Fiat-Shamir hashing: ok_out[i] = 1 when s_i G == A_i + e_i C_i.  commitments A and ct1 C: B G1
+ * affine (device); responses s and challenges e: B canonical scalars each (HOST); ok_out: B
+ * bytes (HOST).  A non-canonical scalar is BTE_EINVAL. */
+int bte_btx_verify_proofs(const bte_p1_affine* commitments, const bte_p1_affine* ct1, const uint8_t* responses,
+                          const uint8_t* challenges, size_t batch_size, uint8_t* ok_out, void* stream);
+
 /* weighted_btx/src/setup.rs:98-233 keygen_with_rng, with the dealer's random choices given:
  * poly = n_coeffs canonical scalars (the degree-t Shamir polynomial, t = n_coeffs - 1), q = N
  * nonzero canonical scalars, party_offsets = N+1 prefix sums of the weights (all HOST).

@@ -622,6 +622,20 @@ def pfe_prepare(material, total_weight, batch_size, party_offsets, accepted, alp
     return d1p, tl, ml
 
 
+def btx_verify_proofs(commitments, ct1_affine, responses, challenges):
+    """SchnorrProof::verify (weighted_btx/src/proof.rs:56-70) for every ciphertext of a batch,
+    given the hashed challenges (canonical ints); returns one bool per ciphertext."""
+    B = len(responses)
+    if len(challenges) != B:
+        raise ValueError("btx_verify_proofs: %d responses for %d challenges" % (B, len(challenges)))
+    if B == 0:
+        return []
+    ok = (ctypes.c_uint8 * B)()
+    call("bte_btx_verify_proofs", _ptr(_dev(commitments)), _ptr(_dev(ct1_affine)), _host_scalars(responses),
+         _host_scalars(challenges), B, ok, _stream())
+    return [bool(ok[i]) for i in range(B)]
+
+
 def pfe_verify_proofs_batched(commitments, ct1_affine, responses, challenges, coefficients):
     """verify_batch (weighted_pfe/src/proof.rs:74-122) given the hashed challenges and the
     verifier's nonzero coefficients (canonical ints)."""

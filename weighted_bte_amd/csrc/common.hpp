@@ -6,6 +6,10 @@
 
 #include "../../include/bte_hip.h"
 
+namespace bteh {
+struct Fr;
+}  // namespace bteh
+
 namespace bte {
 
 // Record the first failure for bte_last_error() and return its status code.
@@ -44,6 +48,10 @@ int g1_msm_batched_gather(const void* points, size_t n_points, const uint32_t* p
 int g2_msm_batched_gather(const void* points, size_t n_points, const uint32_t* point_index, const uint8_t* scalars,
                           const uint32_t* scalar_index, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
                           int c, void* out, hipStream_t s);
+
+// k = k1 + k2 lambda (lambda = |z|^2 - 1, the G1 endomorphism's eigenvalue) as k1[4] then k2[4]
+// little-endian u32 words, the layout g1_glv_mul reads (fft.hip).
+void glv_decompose_g1(const bteh::Fr& k, uint32_t out[8]);
 
 // Segmented cyclotomic prefix inverse transforms (fft.hip).
 struct fp12;

@@ -120,6 +120,13 @@ static void twiddle_decomposition(const bteh::Fr& k, uint32_t out[16]) {
   }
 }
 
+// G1 half only: (k mod lambda, k div lambda) as eight little-endian u32, for g1_glv_mul.
+void glv_decompose_g1(const bteh::Fr& k, uint32_t out[8]) {
+  uint32_t both[16];
+  twiddle_decomposition(k, both);
+  memcpy(out, both, 8 * sizeof(uint32_t));
+}
+
 struct DomainTables {
   uint32_t* dec = nullptr;       // n/2 twiddle decompositions (16 x u32 each)
   uint32_t* twiddles = nullptr;  // n/2 canonical scalars (8 x u32 each): w^j

@@ -6,11 +6,17 @@
 // challenges rho_j: sigma = sum rho_j sigma_j (one G1 MSM) and V_i = sum_j rho_j vk_{i+1,j} (one
 // indexed G2 MSM per valid slot, all in one batched launch sequence), then one multi-pairing.
 // Challenge sampling (OsRng in the reference) stays with the caller.
+//
+// Also the per-ciphertext Schnorr check of weighted_btx/src/proof.rs:56-70 (SchnorrProof::verify)
+// for a whole batch: s_i G == A_i + e_i C_i, with the Fiat-Shamir challenges e_i hashed on the host.
+#include <string.h>
+
 #include <vector>
 
 #include "common.hpp"
 #include "curve.hpp"
 #include "cyclo.hpp"
+#include "host_field.hpp"
 
 extern "C" {
 int bte_g1_msm(const bte_p1_affine* points, const uint8_t* scalars, size_t n, bte_p1* out, void* stream);
@@ -19,6 +25,7 @@ int bte_g2_batch_normalize(const bte_p2* in, size_t n, bte_p2_affine* out, void*
 int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine* q, size_t npairs, const uint64_t* seg_offsets,
                           size_t nseg, bte_fp12* out, void* stream);
 int bte_final_exp(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
+int bte_g1_generator_mul(const uint8_t* scalars, size_t n, bte_p1_affine* out, void* stream);
 }
 
 namespace bte {
@@ -71,6 +78,19 @@ __global__ void k_verify_pairs_folded(const g1a* __restrict__ ct1, const uint32_
 __global__ void k_fp12_is_one(const fp12* __restrict__ v, size_t n, uint8_t* __restrict__ out) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i < n) out[i] = fp12_eq(v[i], fp12_one()) ? 1 : 0;
+}
+
+// ok_i = (s_i G == A_i + e_i C_i): s_i G comes precomputed (fixed-base byte windows), e_i C_i by
+// the G1 endomorphism's joint double-and-add over the host's decomposition of e_i.
+__global__ void __launch_bounds__(64) k_schnorr_check(const g1a* __restrict__ sG, const g1a* __restrict__ A,
+                                                      const g1a* __restrict__ C, const uint32_t* __restrict__ edec,
+                                                      size_t B, uint8_t* __restrict__ ok) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= B) return;
+  jac<fp> d = g1_glv_mul(jac_from_aff(C[i]), edec + 8 * i);
+  d = jac_add_aff(d, A[i]);
+  d = jac_add_aff(d, aff_neg(sG[i]));
+  ok[i] = jac_is_identity(d) ? 1 : 0;
 }
 
 struct VScratch {
@@ -227,5 +247,46 @@ extern "C" int bte_btx_verify_shares_batched(const bte_p1* shares, const uint32_
   BTE_HIP(hipStreamSynchronize(s));
   BTE_CHECK_LAUNCH();
   *result = ok;
+  return BTE_OK;
+}
+
+static bool canonical_fr(const uint8_t* b, bteh::Fr* out) {
+  memcpy(out->l, b, 32);
+  for (int i = 3; i >= 0; --i) {
+    if (out->l[i] < bteh::FR_R[i]) return true;
+    if (out->l[i] > bteh::FR_R[i]) return false;
+  }
+  return false;
+}
+
+extern "C" int bte_btx_verify_proofs(const bte_p1_affine* commitments, const bte_p1_affine* ct1,
+                                     const uint8_t* responses, const uint8_t* challenges, size_t batch_size,
+                                     uint8_t* ok_out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t B = batch_size;
+  if (B == 0) return BTE_OK;
+  if (!commitments || !ct1 || !responses || !challenges || !ok_out)
+    return set_error(BTE_EINVAL, "verify_proofs: null pointer");
+  std::vector<uint32_t> edec(8 * B);
+  for (size_t i = 0; i < B; ++i) {
+    bteh::Fr e, r;
+    if (!canonical_fr(responses + 32 * i, &r) || !canonical_fr(challenges + 32 * i, &e))
+      return set_error(BTE_EINVAL, "verify_proofs: scalar %zu is not canonical", i);
+    glv_decompose_g1(e, &edec[8 * i]);
+  }
+  VScratch sc(s);
+  uint8_t *d_resp, *d_ok;
+  uint32_t* d_edec;
+  g1a* sG;
+  V_TRY(sc.upload(&d_resp, responses, 32 * B));
+  V_TRY(sc.upload(&d_edec, edec.data(), 8 * B));
+  V_TRY(sc.get(&sG, B));
+  V_TRY(sc.get(&d_ok, B));
+  V_TRY(bte_g1_generator_mul(d_resp, B, (bte_p1_affine*)sG, s));
+  hipLaunchKernelGGL(k_schnorr_check, dim3(blocks_for(B, 64)), dim3(64), 0, s, (const g1a*)sG, (const g1a*)commitments,
+                     (const g1a*)ct1, (const uint32_t*)d_edec, B, d_ok);
+  BTE_HIP(hipMemcpyAsync(ok_out, d_ok, B, hipMemcpyDeviceToHost, s));
+  BTE_HIP(hipStreamSynchronize(s));
+  BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
