@@ -308,6 +308,20 @@ int bte_btx_keygen(const uint8_t* poly, size_t n_coeffs, const uint8_t* q, const
                    size_t n_parties, size_t max_batch, bte_p2_affine* material, bte_p2_affine* verification_keys,
                    void* stream);
 
+/* weighted_pfe/src/setup.rs:149-354 keygen with the dealer's random choices given: poly = n_coeffs
+ * canonical scalars (the Shamir polynomial, z = poly(0)); rho = N canonical party labels; alpha =
+ * B canonical alpha points; beta = (beta_{-1}, beta_0), two canonical scalars; party_offsets = N+1
+ * prefix sums of the weights (all HOST).  Outputs: material (device) = d1 (B blocks of W) then d2
+ * (B blocks of W), G2 affine, as bte_pfe_prepare reads it; verification_keys (device, or NULL) =
+ * B x N G2 affine, [g_{j,i}]_2 at i N + j; global_key (device, or NULL) = [z (beta_{-1} +
+ * beta_0)]_2; fractions_out (HOST, or NULL) = N x B canonical g_{j,i} = 1 / (rho_j + alpha_i) at
+ * j B + i.  A zero label rho_j + alpha_i, alpha_i or alpha_i + 1 is BTE_EINVAL (the reference
+ * resamples rho; the caller does here). */
+int bte_pfe_keygen(const uint8_t* poly, size_t n_coeffs, const uint8_t* rho, const uint8_t* alpha, const uint8_t* beta,
+                   const uint64_t* party_offsets, size_t n_parties, size_t batch_size, bte_p2_affine* material,
+                   bte_p2_affine* verification_keys, bte_p2_affine* global_key, uint8_t* fractions_out,
+                   void* stream);
+
 /* decryption.rs:205-233 partial_decrypt for N parties at once: sigma_j = sum_i q_j^i ct1_i,
  * i = 1..B.  q = N canonical scalars (HOST); sigma_out = N G1 Jacobian (device). */
 int bte_btx_partial_decrypt(const bte_p1_affine* ct1, size_t batch_size, const uint8_t* q,

@@ -102,3 +102,25 @@ def test_pfe_proof_batch_verification(api):
     bad = list(resp)
     bad[5] = (bad[5] + 1) % O.R
     assert not api.pfe_verify_proofs_batched(commit, ct1, bad, e, coef)
+
+
+def test_pfe_gpu_keygen_matches_host(api):
+    """bte_pfe_keygen (weighted_pfe/src/setup.rs:149-354 on the device) equals the host-computed
+    scalar tables pushed through the generator: d1 || d2 byte for byte, the global key, the
+    fractions g_{j,i} = 1/(rho_j + alpha_i), and the slot-major verification keys [g_{j,i}]_2."""
+    from weighted_bte_amd import workloads
+    weights = [3, 1, 4, 2, 5]
+    B = 8
+    wl = workloads.PfeWorkload(weights, threshold=6, batch_size=B, seed=33, gpu_keygen=False)
+    mat, gk, g, vk = api.pfe_keygen(wl.poly, wl.rho, wl.alpha, (wl.beta_m1, wl.beta_0), wl.offsets, B,
+                                    verification_keys=True)
+    assert api.to_host(mat) == api.to_host(wl.material)
+    assert api.to_host(gk) == api.to_host(wl.global_key)
+    assert g == wl.g
+    N = len(weights)
+    want_vk = api.g2_generator_mul(api.to_device(b"".join(wl.g[j][i].to_bytes(32, "little")
+                                                          for i in range(B) for j in range(N))))
+    assert api.to_host(vk) == api.to_host(want_vk)
+    for j in range(N):
+        for i in range(B):
+            assert g[j][i] * (wl.rho[j] + wl.alpha[i]) % O.R == 1

@@ -83,6 +83,7 @@ _SIGNATURES = {
     "bte_btx_verify_shares": [_vp, _vp, _sz, _vp, _sz, _sz, _vp, _vp, _sz, _vp, _vp],
     "bte_pfe_verify_proofs_batched": [_vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp],
     "bte_btx_verify_proofs": [_vp, _vp, _vp, _vp, _sz, _vp, _vp],
+    "bte_pfe_keygen": [_vp, _sz, _vp, _vp, _vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp],
     "bte_g1_compress": [_vp, _sz, _vp, _vp],
     "bte_g2_compress": [_vp, _sz, _vp, _vp],
     "bte_g1_decompress": [_vp, _sz, _vp, _vp, _vp],

@@ -525,6 +525,25 @@ def btx_keygen(poly, q, party_offsets, max_batch, verification_keys=False):
     return (mat, vk) if verification_keys else mat
 
 
+def pfe_keygen(poly, rho, alpha, beta, party_offsets, batch_size, verification_keys=False):
+    """weighted_pfe setup.rs:149-354 key material from the dealer's choices (canonical ints):
+    returns (material d1 || d2 on the device, global key on the device, fractions g[j][i] as
+    ints, verification keys on the device or None)."""
+    N, B = len(rho), batch_size
+    W = party_offsets[-1]
+    mat = torch.empty(2 * B * W * G2_AFFINE, dtype=torch.uint8, device="cuda")
+    gk = torch.empty(G2_AFFINE, dtype=torch.uint8, device="cuda")
+    vk = torch.empty(B * N * G2_AFFINE, dtype=torch.uint8, device="cuda") if verification_keys else None
+    frac = ctypes.create_string_buffer(32 * N * B)
+    offs = (ctypes.c_uint64 * len(party_offsets))(*party_offsets)
+    call("bte_pfe_keygen", _host_scalars(poly), len(poly), _host_scalars(rho), _host_scalars(alpha),
+         _host_scalars(beta), offs, N, B, _ptr(mat), _ptr(vk), _ptr(gk), frac, _stream())
+    raw = frac.raw
+    g = [[int.from_bytes(raw[32 * (j * B + i): 32 * (j * B + i) + 32], "little") for i in range(B)]
+         for j in range(N)]
+    return mat, gk, g, vk
+
+
 def btx_verify_shares(shares_jac, party_index, verification_keys, n_parties, max_batch, ct1_affine, valid):
     """verify_decryption_share (decryption.rs:236-272) for every share: a list of bools."""
     T = len(party_index)

@@ -1,6 +1,7 @@
-// weighted_btx key material on the GPU: the scalar tables and fixed-base G2 multiplications of
-// the trusted-dealer setup (weighted_btx/src/setup.rs:98-233), given the dealer's random choices
-// (the Shamir polynomial and every party's q) so that the output is a pure function of them.
+// Key material on the GPU: the scalar tables and fixed-base G2 multiplications of the
+// trusted-dealer setups of weighted_btx (setup.rs:98-233) and weighted_pfe (setup.rs:149-354),
+// given the dealer's random choices (the Shamir polynomial, every party's q or rho, pfe's alpha
+// and beta) so that the output is a pure function of them.
 //
 //   z_v = poly(x_v)                      x_v = omega^v, the first W points of the radix-2 domain
 //                                        of size nextpow2(W) (setup.rs:111-131)
@@ -170,6 +171,144 @@ extern "C" int bte_btx_keygen(const uint8_t* poly, size_t n_coeffs, const uint8_
                          d_vk);
       rc = bte_g2_generator_mul((const uint8_t*)d_vk, N * L, verification_keys, s);
     }
+  } while (0);
+  for (void* p : bufs) (void)hipFreeAsync(p, s);
+  if (rc) return rc;
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+// weighted_pfe (setup.rs:149-354): g_{j,i} = 1 / (rho_j + alpha_i), u_{j,i} = g_{j,i}^2 +
+// beta_{-1} / (alpha_i + 1) + beta_0 / alpha_i (host, one batched inversion each), then on the
+// device d1 block i = [z_v g_{owner(v),i}]_2 and d2 block i = [z_v u_{owner(v),i}]_2 over the W
+// share points, verification keys [g_{j,i}]_2 slot-major and the global key [z (beta_{-1} +
+// beta_0)]_2 with z = poly(0).
+static bool batch_invert(std::vector<bteh::Fr>& v) {
+  std::vector<bteh::Fr> pre(v.size());
+  bteh::Fr acc = bteh::fr_from_u64(1);
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (bteh::fr_is_zero(v[i])) return false;
+    pre[i] = acc;
+    acc = bteh::fr_mul(acc, v[i]);
+  }
+  bteh::Fr inv = bteh::fr_inv(acc);
+  for (size_t i = v.size(); i-- > 0;) {
+    const bteh::Fr vi = v[i];
+    v[i] = bteh::fr_mul(inv, pre[i]);
+    inv = bteh::fr_mul(inv, vi);
+  }
+  return true;
+}
+
+extern "C" int bte_pfe_keygen(const uint8_t* poly, size_t n_coeffs, const uint8_t* rho, const uint8_t* alpha,
+                              const uint8_t* beta, const uint64_t* party_offsets, size_t n_parties, size_t batch_size,
+                              bte_p2_affine* material, bte_p2_affine* verification_keys, bte_p2_affine* global_key,
+                              uint8_t* fractions_out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t N = n_parties, B = batch_size;
+  if (!poly || !rho || !alpha || !beta || !party_offsets || !material || n_coeffs == 0 || N == 0 || B == 0)
+    return set_error(BTE_EINVAL, "pfe_keygen: bad arguments");
+  const size_t W = party_offsets[N];
+  if (party_offsets[0] != 0 || W == 0 || W > (1u << 28)) return set_error(BTE_EINVAL, "pfe_keygen: bad party offsets");
+  for (size_t j = 0; j < N; ++j)
+    if (party_offsets[j + 1] <= party_offsets[j]) return set_error(BTE_EINVAL, "pfe_keygen: party %zu has no weight", j);
+  for (size_t k = 0; k < n_coeffs; ++k)
+    if (!fr_canonical(poly + 32 * k)) return set_error(BTE_EINVAL, "pfe_keygen: coefficient %zu not canonical", k);
+  for (size_t j = 0; j < N; ++j)
+    if (!fr_canonical(rho + 32 * j)) return set_error(BTE_EINVAL, "pfe_keygen: rho %zu not canonical", j);
+  for (size_t i = 0; i < B; ++i)
+    if (!fr_canonical(alpha + 32 * i)) return set_error(BTE_EINVAL, "pfe_keygen: alpha %zu not canonical", i);
+  if (!fr_canonical(beta) || !fr_canonical(beta + 32)) return set_error(BTE_EINVAL, "pfe_keygen: beta not canonical");
+  bteh::Fr bm1, b0;
+  memcpy(bm1.l, beta, 32);
+  memcpy(b0.l, beta + 32, 32);
+  std::vector<bteh::Fr> al(B), g(N * B), ai(B), a1i(B);
+  for (size_t i = 0; i < B; ++i) {
+    memcpy(al[i].l, alpha + 32 * i, 32);
+    ai[i] = al[i];
+    a1i[i] = bteh::fr_add(al[i], bteh::fr_from_u64(1));
+  }
+  for (size_t j = 0; j < N; ++j) {
+    bteh::Fr rj;
+    memcpy(rj.l, rho + 32 * j, 32);
+    for (size_t i = 0; i < B; ++i) g[j * B + i] = bteh::fr_add(rj, al[i]);
+  }
+  if (!batch_invert(g)) return set_error(BTE_EINVAL, "pfe_keygen: a label rho_j + alpha_i is zero");
+  if (!batch_invert(ai) || !batch_invert(a1i)) return set_error(BTE_EINVAL, "pfe_keygen: alpha_i or alpha_i + 1 is zero");
+  std::vector<bteh::Fr> gT(B * N), uT(B * N);  // slot-major: [i N + j]
+  for (size_t j = 0; j < N; ++j)
+    for (size_t i = 0; i < B; ++i) {
+      const bteh::Fr gj = g[j * B + i];
+      gT[i * N + j] = gj;
+      uT[i * N + j] = bteh::fr_add(bteh::fr_mul(gj, gj),
+                                   bteh::fr_add(bteh::fr_mul(bm1, a1i[i]), bteh::fr_mul(b0, ai[i])));
+    }
+  if (fractions_out) memcpy(fractions_out, g.data(), 32 * N * B);
+  int logd = 0;
+  while (((size_t)1 << logd) < W) ++logd;
+  bteh::Fr w;
+  memcpy(w.l, bteh::FR_ROOT_OF_UNITY, 32);
+  for (int i = logd; i < bteh::FR_S; ++i) w = bteh::fr_mul(w, w);
+  std::vector<bteh::Fr> xs(W);
+  xs[0] = bteh::fr_from_u64(1);
+  for (size_t v = 1; v < W; ++v) xs[v] = bteh::fr_mul(xs[v - 1], w);
+  std::vector<uint32_t> owner(W);
+  for (size_t j = 0; j < N; ++j)
+    for (uint64_t v = party_offsets[j]; v < party_offsets[j + 1]; ++v) owner[v] = (uint32_t)j;
+  bteh::Fr z0, gk;
+  memcpy(z0.l, poly, 32);
+  gk = bteh::fr_mul(z0, bteh::fr_add(bm1, b0));
+
+  fr *d_coef, *d_x, *d_z, *d_gc, *d_gm, *d_um, *d_sc, *d_gk;
+  uint32_t* d_owner;
+  const size_t chunk_pts = std::max<size_t>(W, (size_t)1 << 20);
+  const size_t blk_per_chunk = std::max<size_t>(1, chunk_pts / W);
+  std::vector<void*> bufs;
+  auto get = [&](void** p, size_t bytes) -> int {
+    hipError_t e = hipMallocAsync(p, bytes ? bytes : 1, s);
+    if (e != hipSuccess) return set_error(BTE_ENOMEM, "pfe_keygen: %s", hipGetErrorString(e));
+    bufs.push_back(*p);
+    return BTE_OK;
+  };
+  int rc = BTE_OK;
+  do {
+    if ((rc = get((void**)&d_coef, 32 * n_coeffs)) || (rc = get((void**)&d_x, 32 * W)) ||
+        (rc = get((void**)&d_z, 32 * W)) || (rc = get((void**)&d_gc, 32 * B * N)) ||
+        (rc = get((void**)&d_gm, 32 * B * N)) || (rc = get((void**)&d_um, 32 * B * N)) ||
+        (rc = get((void**)&d_sc, 32 * blk_per_chunk * W)) || (rc = get((void**)&d_gk, 32)) ||
+        (rc = get((void**)&d_owner, 4 * W)))
+      break;
+    if (hipMemcpyAsync(d_coef, poly, 32 * n_coeffs, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(d_x, xs.data(), 32 * W, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(d_gc, gT.data(), 32 * B * N, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(d_gm, gT.data(), 32 * B * N, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(d_um, uT.data(), 32 * B * N, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(d_gk, gk.l, 32, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(d_owner, owner.data(), 4 * W, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      rc = set_error(BTE_EDEVICE, "pfe_keygen: upload failed");
+      break;
+    }
+    hipLaunchKernelGGL(k_fr_to_mont, dim3(blocks_for(n_coeffs, 256)), dim3(256), 0, s, d_coef, n_coeffs);
+    hipLaunchKernelGGL(k_fr_to_mont, dim3(blocks_for(W, 256)), dim3(256), 0, s, d_x, W);
+    hipLaunchKernelGGL(k_fr_to_mont, dim3(blocks_for(B * N, 256)), dim3(256), 0, s, d_gm, B * N);
+    hipLaunchKernelGGL(k_fr_to_mont, dim3(blocks_for(B * N, 256)), dim3(256), 0, s, d_um, B * N);
+    hipLaunchKernelGGL(k_poly_eval, dim3(blocks_for(W, 256)), dim3(256), 0, s, (const fr*)d_coef, n_coeffs,
+                       (const fr*)d_x, W, d_z);
+    // d1 blocks 0..B-1 (g), then d2 blocks 0..B-1 (u)
+    size_t b0i = 0;
+    while (b0i < 2 * B && !rc) {
+      const bool first = b0i < B;
+      const size_t nb = std::min(blk_per_chunk, (first ? B : 2 * B) - b0i);
+      hipLaunchKernelGGL(k_material_scalars, dim3(blocks_for(nb * W, 256)), dim3(256), 0, s, (const fr*)d_z,
+                         (const uint32_t*)d_owner, W, (const fr*)(first ? d_gm : d_um), N, first ? b0i : b0i - B, nb,
+                         d_sc);
+      rc = bte_g2_generator_mul((const uint8_t*)d_sc, nb * W, material + b0i * W, s);
+      b0i += nb;
+    }
+    if (rc) break;
+    if (verification_keys && (rc = bte_g2_generator_mul((const uint8_t*)d_gc, B * N, verification_keys, s))) break;
+    if (global_key && (rc = bte_g2_generator_mul((const uint8_t*)d_gk, 1, global_key, s))) break;
   } while (0);
   for (void* p : bufs) (void)hipFreeAsync(p, s);
   if (rc) return rc;

@@ -117,7 +117,7 @@ class PfeWorkload:
     power-of-two batch size B: device G2 array d1 (B blocks of W) then d2 (B blocks of W), the
     global key V, alpha points and party fractions g_{j,i} = 1/(rho_j + alpha_i)."""
 
-    def __init__(self, weights, threshold, batch_size, seed=1):
+    def __init__(self, weights, threshold, batch_size, seed=1, gpu_keygen=True):
         st = inputs.Stream(seed, 5000)
         B = batch_size
         self.weights = list(weights)
@@ -161,22 +161,34 @@ class PfeWorkload:
                 continue
             labels.update(lab)
             self.rho.append(rho)
-        self.g = [[pow((rho + a) % R, -1, R) for a in self.alpha] for rho in self.rho]
-        ainv = [pow(a, -1, R) for a in self.alpha]
-        a1inv = [pow(a + 1, -1, R) for a in self.alpha]
-        u = [[(gj[i] * gj[i] + self.beta_m1 * a1inv[i] + self.beta_0 * ainv[i]) % R for i in range(B)]
-             for gj in self.g]
-        d1 = b"".join((zv[v] * self.g[owner[v]][i] % R).to_bytes(32, "little")
-                      for i in range(B) for v in range(self.W))
-        d2 = b"".join((zv[v] * u[owner[v]][i] % R).to_bytes(32, "little")
-                      for i in range(B) for v in range(self.W))
-        self.material = api.g2_generator_mul(api.to_device(d1 + d2))
-        vkey = self.z * (self.beta_m1 + self.beta_0) % R
-        self.global_key = api.g2_generator_mul(api.to_device(vkey.to_bytes(32, "little")))
+        self.poly = poly
+        if gpu_keygen:
+            self.material, self.global_key, self.g, _ = api.pfe_keygen(
+                poly, self.rho, self.alpha, (self.beta_m1, self.beta_0), self.offsets, B)
+        else:
+            self.material, self.global_key, self.g = self.host_material(zv, owner)
         self.zv = zv
         self.owner = owner
         self.stream = st
         self.gt = gt_generator_bytes()
+
+    def host_material(self, zv, owner):
+        """The key material with the scalar tables computed on the host (the check of the device
+        key generation): (d1 || d2, global key, fractions g[j][i])."""
+        B = self.B
+        g = [[pow((rho + a) % R, -1, R) for a in self.alpha] for rho in self.rho]
+        ainv = [pow(a, -1, R) for a in self.alpha]
+        a1inv = [pow(a + 1, -1, R) for a in self.alpha]
+        u = [[(gj[i] * gj[i] + self.beta_m1 * a1inv[i] + self.beta_0 * ainv[i]) % R for i in range(B)]
+             for gj in g]
+        d1 = b"".join((zv[v] * g[owner[v]][i] % R).to_bytes(32, "little")
+                      for i in range(B) for v in range(self.W))
+        d2 = b"".join((zv[v] * u[owner[v]][i] % R).to_bytes(32, "little")
+                      for i in range(B) for v in range(self.W))
+        material = api.g2_generator_mul(api.to_device(d1 + d2))
+        vkey = self.z * (self.beta_m1 + self.beta_0) % R
+        global_key = api.g2_generator_mul(api.to_device(vkey.to_bytes(32, "little")))
+        return material, global_key, g
 
     def encrypt(self, count=1):
         """count batches of B messages gT^s with ct1 = r G1 and ct2 = gT^(s + z beta_0 r)
