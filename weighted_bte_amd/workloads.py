@@ -132,17 +132,11 @@ class PfeWorkload:
         dom = 1
         while dom < self.W:
             dom <<= 1
-        g = pow(ROOT_OF_UNITY, (1 << 32) // dom, R)
-        xs = [pow(g, v, R) for v in range(self.W)]
+        self._omega = pow(ROOT_OF_UNITY, (1 << 32) // dom, R)
         poly = st.scalars(threshold + 1)
         poly[0] = poly[0] or 1
         self.z = poly[0]
-        zv = []
-        for x in xs:
-            acc = 0
-            for c in reversed(poly):
-                acc = (acc * x + c) % R
-            zv.append(acc)
+        self._zv = None
         # alpha_i = omega_B^i + c with the smallest c >= 2 (setup.rs:605-618)
         gb = pow(ROOT_OF_UNITY, (1 << 32) // B, R)
         roots = [pow(gb, i, R) for i in range(B)]
@@ -166,11 +160,25 @@ class PfeWorkload:
             self.material, self.global_key, self.g, _ = api.pfe_keygen(
                 poly, self.rho, self.alpha, (self.beta_m1, self.beta_0), self.offsets, B)
         else:
-            self.material, self.global_key, self.g = self.host_material(zv, owner)
-        self.zv = zv
+            self.material, self.global_key, self.g = self.host_material(self.zv, owner)
         self.owner = owner
         self.stream = st
         self.gt = gt_generator_bytes()
+
+    @property
+    def zv(self):
+        """Share values z_v = poly(omega^v) at the W virtual points (host, computed on first use:
+        the device key generation evaluates them itself)."""
+        if self._zv is None:
+            zv = []
+            for v in range(self.W):
+                x = pow(self._omega, v, R)
+                acc = 0
+                for c in reversed(self.poly):
+                    acc = (acc * x + c) % R
+                zv.append(acc)
+            self._zv = zv
+        return self._zv
 
     def host_material(self, zv, owner):
         """The key material with the scalar tables computed on the host (the check of the device
