@@ -488,7 +488,7 @@ def main():
     ap.add_argument("--logn", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="time the built-in window table only")
-    ap.add_argument("--no-config2", action="store_true",
+    ap.add_argument("--no-sizes", action="store_true",
                     help="skip the MSM size table, the FFT size table and the uneven multi-pairing")
     ap.add_argument("--no-pfe", action="store_true", help="skip the weighted_pfe decryption measurement")
     ap.add_argument("--no-serving-large", action="store_true",
@@ -577,9 +577,9 @@ def main():
     nw = (256 + c - 1) // c
     msm_fp_mul = nw * (11 * n + 32 * (1 << c)) + 7 * 255
     mac_peak = W.probe_mac_rate(torch.cuda.current_stream().cuda_stream)
-    config2 = None if a.no_config2 else bench_msm_sizes(not a.no_tune)
-    ffts = None if a.no_config2 else bench_fft_sizes()
-    mpair = None if a.no_config2 else bench_multi_pairing()
+    msm_sweep = None if a.no_sizes else bench_msm_sizes(not a.no_tune)
+    ffts = None if a.no_sizes else bench_fft_sizes()
+    mpair = None if a.no_sizes else bench_multi_pairing()
     pfe = None if a.no_pfe else bench_pfe()
 
     cpu = None if a.no_cpu else cpu_baseline(a.logn)
@@ -624,7 +624,7 @@ def main():
             "mixed_adds_per_launch": E,
         },
         "cpu_baseline": cpu,
-        "msm_sizes": config2,
+        "msm_sizes": msm_sweep,
         "fft_sizes": ffts,
         "multi_pairing": mpair,
         "pfe_decrypt": pfe,
@@ -634,7 +634,7 @@ def main():
     }
     print(json.dumps(line))
     ok = parity and (decrypt is None or decrypt["plaintexts_bit_exact"]) and \
-        (config2 is None or all(r["parity"] for r in config2)) and \
+        (msm_sweep is None or all(r["parity"] for r in msm_sweep)) and \
         (ffts is None or all(r["roundtrip_ok"] for r in ffts)) and \
         (mpair is None or mpair["parity"]) and (pfe is None or pfe["plaintexts_bit_exact"]) and \
         (serving is None or serving["plaintexts_bit_exact"]) and \

@@ -99,7 +99,7 @@ def test_btx_beta_matches_literal_formula_and_decrypts(api, B):
     assert got == [O.f12_pow(g, s) for s in st.s]
 
 
-def test_btx_config1_shape_with_invalid_slot(api):
+def test_btx_small_committee_with_invalid_slot(api):
     """n = 8 parties, weights U[1,8] from the seed, threshold = W // 2, B = L = 32; slot 5 is
     invalid (identity first component) and opens to None (one)."""
     rng = O.SeededRng(11, 1)

@@ -182,7 +182,7 @@ def test_g1_fft_segmented_uneven_sizes(api):
     assert back == [g1_mul_gen(k) for k in flat]
 
 
-def test_config3_batched_64_ffts_log_uniform(api):
+def test_batched_64_ffts_log_uniform_sizes(api):
     """64 independent G1 FFTs of log-uniform sizes 2^6..2^12 in one launch per stage: round trip
     and closed-form spot checks of forward outputs."""
     import math

@@ -131,8 +131,8 @@ def test_generator_mul_and_batch_normalize(api):
 
 
 @pytest.mark.parametrize("logn", [12, 16])
-def test_g1_msm_closed_form_config_sizes(api, logn):
-    """BASELINE config-2 shape: bases k_i*G from the seed, 90% uniform + 10% sparse scalars;
+def test_g1_msm_closed_form_bench_sizes(api, logn):
+    """The standalone-MSM benchmark shape: bases k_i*G from the seed, 90% uniform + 10% sparse scalars;
     MSM == (sum k_i s_i mod r) * G, compared as compressed bytes."""
     n = 1 << logn
     rng = O.SeededRng(2, logn)
