@@ -168,6 +168,104 @@ BTE_DEV fp fp_mul(const fp& a, const fp& b) {
 
 BTE_DEV fp fp_sqr(const fp& a) { return fp_mul(a, a); }
 
+// ---- double-width products for lazy reduction ---------------------------------------------------
+// A 768-bit value as 24 little-endian limbs: a product of two elements below 2p, or a sum or
+// difference of such products, reduced once by fp_redc_wide.
+struct fpw {
+  uint32_t l[24];
+};
+
+// p^2 (24 x u32 LE).
+#define BTE_P2_LIMBS                                                                             \
+  0x1c718e39u, 0x26aa0000u, 0x76382eabu, 0x7ced6b1du, 0x62113cfdu, 0x162c3383u, 0x3e71b743u,    \
+      0x66bf91edu, 0x7091a049u, 0x292e85a8u, 0x86185c7bu, 0x1d68619cu, 0x0978ef01u, 0xf5314933u, \
+      0x16ddca6eu, 0x50a62cfdu, 0x349e8bd0u, 0x66e59e49u, 0x0e7046b4u, 0xe2dc90e5u, 0xa22f25e9u, \
+      0x4bd278eau, 0xb8c35fc7u, 0x02a437a4u
+
+// a + b without reduction (a, b < p: the sum is below 2p < 2^382).
+BTE_DEV fp fp_add_raw(const fp& a, const fp& b) {
+  fp s;
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) s.l[i] = __builtin_addc(a.l[i], b.l[i], c, &c);
+  return s;
+}
+
+// The full 768-bit product a * b (product scanning, no reduction).
+BTE_DEV fpw fp_mul_wide(const fp& a, const fp& b) {
+  fpw r;
+  uint64_t acc = 0;
+  uint32_t hi = 0;
+#pragma unroll
+  for (int k = 0; k < 23; ++k) {
+    const int ilo = k < 12 ? 0 : k - 11;
+    const int ihi = k < 12 ? k : 11;
+#pragma unroll
+    for (int i = ilo; i <= ihi; ++i) mac(acc, hi, a.l[i], b.l[k - i]);
+    r.l[k] = (uint32_t)acc;
+    acc = (acc >> 32) | ((uint64_t)hi << 32);
+    hi = 0;
+  }
+  r.l[23] = (uint32_t)acc;
+  return r;
+}
+
+BTE_DEV fpw fpw_add(const fpw& a, const fpw& b) {
+  fpw r;
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 24; ++i) r.l[i] = __builtin_addc(a.l[i], b.l[i], c, &c);
+  return r;
+}
+
+// a - b for a >= b.
+BTE_DEV fpw fpw_sub(const fpw& a, const fpw& b) {
+  fpw r;
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 24; ++i) r.l[i] = __builtin_subc(a.l[i], b.l[i], borrow, &borrow);
+  return r;
+}
+
+// p^2 - b for b < p^2.
+BTE_DEV fpw fpw_p2_sub(const fpw& b) {
+  constexpr uint32_t P2[24] = {BTE_P2_LIMBS};
+  fpw r;
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 24; ++i) r.l[i] = __builtin_subc(P2[i], b.l[i], borrow, &borrow);
+  return r;
+}
+
+// Montgomery reduction T 2^-384 mod p for T < p 2^384 (product scanning; the value before the
+// final subtraction is below 2p).
+BTE_DEV fp fp_redc_wide(const fpw& T) {
+  uint32_t m[12];
+  fp r;
+  uint64_t acc = 0;
+  uint32_t hi = 0;
+#pragma unroll
+  for (int k = 0; k < 24; ++k) {
+    {  // acc += T_k
+      const uint64_t s = acc + T.l[k];
+      hi += s < acc;
+      acc = s;
+    }
+    const int ilo = k < 12 ? 0 : k - 11;
+#pragma unroll
+    for (int i = ilo; i < (k < 12 ? k : 12); ++i) mac_s(acc, hi, m[i], p_limb(k - i));
+    if (k < 12) {
+      m[k] = (uint32_t)acc * BTE_PINV;
+      mac_s(acc, hi, m[k], p_limb(0));
+    } else {
+      r.l[k - 12] = (uint32_t)acc;
+    }
+    acc = (acc >> 32) | ((uint64_t)hi << 32);
+    hi = 0;
+  }
+  return fp_reduce_once(r);
+}
+
 // Small-constant multiples used by the curve formulas.
 BTE_DEV fp fp_mul3(const fp& a) { return fp_add(fp_dbl(a), a); }
 BTE_DEV fp fp_mul4(const fp& a) { return fp_dbl(fp_dbl(a)); }
