@@ -166,8 +166,12 @@ BTE_DEV fp4 c3_tab_get(const c3tab& t, int e) {
   return {{x[0], x[1]}, {x[2], x[3]}};
 }
 
-// The decomposed action of a plan (cyclo_pow_plan) on a three-lane element; the odd powers
-// v, v^3, v^5, v^7 live in the spill area.  Same result as cyclo_pow_plan.
+// The decomposed action of a plan (cyclo_pow_plan) on a three-lane element; the table
+// phi_comp(v^(2i+1)) (conjugated for odd comp) of C3_TAB_ENTRIES entries lives in the spill area,
+// so each Frobenius image is taken once per scalar.  Same result as cyclo_pow_plan.
+static constexpr int C3_TAB_ENTRIES = 16;
+static constexpr size_t C3_TAB_BYTES = (size_t)C3_TAB_ENTRIES * 48 * sizeof(uint32_t);  // per lane
+
 BTE_DEV fp4 c3_cyclo_pow_plan(const c3ctx& c, const c3tab& t, const fp4& v, const cyc_plan* __restrict__ plan) {
   const int count = plan->count;
   if (count == 0) return c3_one(c);
@@ -179,6 +183,14 @@ BTE_DEV fp4 c3_cyclo_pow_plan(const c3ctx& c, const c3tab& t, const fp4& v, cons
     cur = c3_mul(c, cur, two);
     c3_tab_put(t, i, cur);
   }
+#pragma unroll 1
+  for (int comp = 1; comp < 4; ++comp)
+#pragma unroll 1
+    for (int i = 0; i < 4; ++i) {
+      fp4 m = c3_frobenius(c, c3_tab_get(t, i), comp);
+      if (comp & 1) m = c3_conj(c, m);
+      c3_tab_put(t, 4 * comp + i, m);
+    }
   fp4 acc = c3_one(c);
   bool started = false;
 #pragma unroll 1
@@ -188,9 +200,8 @@ BTE_DEV fp4 c3_cyclo_pow_plan(const c3ctx& c, const c3tab& t, const fp4& v, cons
     for (int comp = 0; comp < 4; ++comp) {
       const int d = plan->naf[comp][pos];
       if (d == 0) continue;
-      fp4 m = c3_tab_get(t, ((d < 0 ? -d : d) - 1) >> 1);
-      if (comp) m = c3_frobenius(c, m, comp);
-      if ((comp & 1) != (d < 0)) m = c3_conj(c, m);
+      fp4 m = c3_tab_get(t, 4 * comp + (((d < 0 ? -d : d) - 1) >> 1));
+      if (d < 0) m = c3_conj(c, m);  // a negative digit inverts (conjugates)
       acc = started ? c3_mul(c, acc, m) : m;
       started = true;
     }

@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(64) k_group_scale(jac<F>* __restrict__ v, size
 // Cyclotomic stages come in two forms.  One thread per butterfly (k_cyc_stage, ...) is the
 // faster per element once a stage has enough butterflies to fill the GPU; below that, each
 // element runs on three lanes (coop12.hpp, *3 kernels: 21 butterflies per 64-lane workgroup,
-// `tab` holding the per-lane odd powers of the plan exponentiation, 768 bytes per lane of the
+// `tab` holding the per-lane exponentiation table, C3_TAB_BYTES per lane of the
 // grid), which shortens the per-stage latency of small transforms.  A stage with fewer than
 // C3_MAX_BUTTERFLIES butterflies takes the three-lane form.
 static constexpr int C3_PER_WG = 21;
@@ -374,7 +374,7 @@ static int group_fft(jac<F>* v, size_t n, size_t count, bool inverse, bool scale
 // need (zero when they run one thread per butterfly).
 size_t cyc_tab_bytes(size_t n, size_t count) {
   const size_t nbf = n / 2 * count;
-  return nbf && nbf < C3_MAX_BUTTERFLIES ? (size_t)c3_blocks(nbf) * 64 * 768 : 0;
+  return nbf && nbf < C3_MAX_BUTTERFLIES ? (size_t)c3_blocks(nbf) * 64 * C3_TAB_BYTES : 0;
 }
 
 // mode 0: full transform; mode 1: prefix inverse (last stage only for `out_count` outputs).
@@ -392,8 +392,8 @@ static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool sc
   if (rc) return rc;
   const size_t nbf = n / 2 * count;
   const bool three = nbf < C3_MAX_BUTTERFLIES;
-  uint32_t* tab = ext_tab;  // odd powers of the plan exponentiations, 768 bytes per lane
-  if (three && !tab) BTE_HIP(hipMallocAsync((void**)&tab, (size_t)c3_blocks(nbf) * 64 * 768, s));
+  uint32_t* tab = ext_tab;  // exponentiation tables, C3_TAB_BYTES per lane
+  if (three && !tab) BTE_HIP(hipMallocAsync((void**)&tab, (size_t)c3_blocks(nbf) * 64 * C3_TAB_BYTES, s));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse<fp12>), dim3(blocks_for(n * count, 256)), dim3(256), 0, s, v, n,
                      logn, count);
   const int full_stages = mode == 1 ? logn - 1 : logn;
@@ -659,7 +659,7 @@ int cyc_prefix_ifft_segmented(fp12* v, const int* log_sizes, const uint32_t* out
     if (nbf < C3_MAX_BUTTERFLIES) widest3 = std::max(widest3, nbf);
   }
   uint32_t* tab = nullptr;
-  if (widest3) BTE_HIP(hipMallocAsync((void**)&tab, (size_t)c3_blocks(widest3) * 64 * 768, s));
+  if (widest3) BTE_HIP(hipMallocAsync((void**)&tab, (size_t)c3_blocks(widest3) * 64 * C3_TAB_BYTES, s));
   BTE_HIP(hipStreamSynchronize(s));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse_seg<fp12>), dim3(blocks_for(total, 256)), dim3(256), 0, s, v,
                      (const uint64_t*)d_off, (const uint8_t*)d_logs, (uint32_t)count, total);
