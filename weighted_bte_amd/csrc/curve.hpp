@@ -196,11 +196,102 @@ BTE_DEV jac<fp2> g2_gls_mul(const jac<fp2>& p, const uint32_t* __restrict__ k) {
   return jac_mul2(p, g2_psi(g2_psi(p)), k, k + 4);
 }
 
-// Multiplication by an FFT twiddle from its precomputed two-part decomposition (8 x u32 for G1,
-// then 8 x u32 for G2: 16 words per twiddle).
+// Multiplication by an FFT twiddle from its two-part decomposition (8 x u32 for G1, then 8 x u32
+// for G2: 16 words per twiddle) by the joint binary double-and-add: the shorter per-butterfly
+// chain for transforms too small to fill the device.
 BTE_DEV jac<fp> twiddle_mul(const jac<fp>& p, const uint32_t* __restrict__ dec) { return g1_glv_mul(p, dec); }
 BTE_DEV jac<fp2> twiddle_mul(const jac<fp2>& p, const uint32_t* __restrict__ dec) {
   return g2_gls_mul(p, dec + 8);
+}
+BTE_DEV const uint32_t* twiddle_at(const uint32_t* dec, size_t k) { return dec + 16 * k; }
+
+// Multiplication by an FFT twiddle k = k1 + k2 lambda (G1, lambda = |z|^2 - 1, endomorphism phi)
+// or k = k1 + k2 |z|^2 (G2, psi^2), from host-built width-3 NAF digits of k1 and k2 (0, +-1, +-3,
+// least significant first): one shared chain of at most 129 doublings and about a quarter as many
+// additions per component, with P and 3P as the only precomputed points (the endomorphism is
+// applied to them when a k2 digit uses them).  A third fewer additions than the binary form, at
+// a longer chain per addition: the form for transforms that fill the device.
+static constexpr int TW_NAF_LEN = 130;
+static constexpr int TW_NAF_BLOCKS = 5;  // 16-byte blocks of 32 four-bit digits
+struct tw_naf {
+  // [group: 0 G1, 1 G2][component: k1, k2]: digit i (two's complement, 4 bits) in bits
+  // 4 (i mod 8) .. of word i / 8
+  uint32_t d[2][2][TW_NAF_BLOCKS * 4];
+  int16_t top[2];  // highest nonzero position per group, -1 when k = 0
+  int16_t pad[6];
+};
+static_assert(sizeof(tw_naf) % 16 == 0, "tw_naf arrays keep 16-byte digit blocks aligned");
+BTE_DEV const tw_naf* twiddle_at(const tw_naf* t, size_t k) { return t + k; }
+
+BTE_DEV uint32_t tw_word(const uint4& b, int j) { return j == 0 ? b.x : j == 1 ? b.y : j == 2 ? b.z : b.w; }
+BTE_DEV int tw_digit(const uint4& b, int i) {
+  const uint32_t nib = (tw_word(b, (i >> 3) & 3) >> (4 * (i & 7))) & 15u;
+  return ((int)(nib << 28)) >> 28;
+}
+
+// The digit blocks are read 32 digits at a time, the next block fetched one block ahead, so a
+// lone wavefront does not wait on a load per digit.
+template <class F, class Endo>
+BTE_DEV jac<F> jac_mul2_naf(const jac<F>& p, const uint32_t* __restrict__ d1, const uint32_t* __restrict__ d2,
+                            int top, Endo endo) {
+  jac<F> acc = jac_identity<F>();
+  if (top < 0) return acc;
+  const uint4* b1 = reinterpret_cast<const uint4*>(d1);
+  const uint4* b2 = reinterpret_cast<const uint4*>(d2);
+  int blk = top >> 5;
+  uint4 c1 = b1[blk], c2 = b2[blk];
+  uint4 n1 = c1, n2 = c2;
+  if (blk) {
+    n1 = b1[blk - 1];
+    n2 = b2[blk - 1];
+  }
+  const jac<F> p3 = jac_add(jac_dbl(p), p);
+#pragma unroll 1
+  for (int i = top; i >= 0; --i) {
+    if ((i & 31) == 31 && i != top) {  // entering block i / 32
+      c1 = n1;
+      c2 = n2;
+      blk = i >> 5;
+      if (blk) {
+        n1 = b1[blk - 1];
+        n2 = b2[blk - 1];
+      }
+    }
+    acc = jac_dbl(acc);
+    const int a = tw_digit(c1, i), b = tw_digit(c2, i);
+    if (a) {
+      const jac<F> t = (a == 1 || a == -1) ? p : p3;
+      acc = jac_add(acc, a < 0 ? jac_neg(t) : t);
+    }
+    if (b) {
+      const jac<F> t = endo((b == 1 || b == -1) ? p : p3);
+      acc = jac_add(acc, b < 0 ? jac_neg(t) : t);
+    }
+  }
+  return acc;
+}
+
+BTE_DEV jac<fp> twiddle_mul(const jac<fp>& p, const tw_naf* __restrict__ t) {
+  fp beta;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) beta.l[i] = G1_BETA[i];
+  return jac_mul2_naf(p, t->d[0][0], t->d[0][1], t->top[0],
+                      [&](const jac<fp>& q) { return jac<fp>{fp_mul(q.x, beta), q.y, q.z}; });
+}
+
+// psi^2(X, Y, Z) = (N(c_x) X, N(c_y) Y, Z) with the norms N(c) = conj(c) c in Fp of psi's constants.
+BTE_DEV jac<fp2> twiddle_mul(const jac<fp2>& p, const tw_naf* __restrict__ t) {
+  fp2 cx, cy;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    cx.c0.l[i] = G2_PSI[0][0][i];
+    cx.c1.l[i] = G2_PSI[0][1][i];
+    cy.c0.l[i] = G2_PSI[1][0][i];
+    cy.c1.l[i] = G2_PSI[1][1][i];
+  }
+  const fp nx = fp2_mul(fp2_conj(cx), cx).c0, ny = fp2_mul(fp2_conj(cy), cy).c0;
+  return jac_mul2_naf(p, t->d[1][0], t->d[1][1], t->top[1],
+                      [&](const jac<fp2>& q) { return jac<fp2>{fp2_mul_fp(q.x, nx), fp2_mul_fp(q.y, ny), q.z}; });
 }
 
 // Left-to-right double-and-add by a canonical scalar (8 x u32, below 2^255).

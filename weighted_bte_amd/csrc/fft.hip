@@ -127,8 +127,52 @@ void glv_decompose_g1(const bteh::Fr& k, uint32_t out[8]) {
   memcpy(out, both, 8 * sizeof(uint32_t));
 }
 
+// Width-3 NAF of a value below 2^128 as four-bit digits, least significant first (digit i in bits
+// 4 (i mod 8) of word i / 8); returns the highest nonzero position (-1 for zero).
+// A group FFT stage uses the NAF twiddle form when every twiddle of the stage serves at least a
+// wavefront of butterflies (twiddle-major numbering then gives each wavefront one digit pattern);
+// otherwise lanes with different twiddles would take both of the NAF step's additions, and the
+// binary form (at most one addition per step) is faster.
+static bool naf_stage(size_t butterflies, int log_half) { return butterflies >> log_half >= 64; }
+
+static int wnaf3(unsigned __int128 k, uint32_t* words) {
+  int top = -1;
+  for (int i = 0; i < TW_NAF_BLOCKS * 4; ++i) words[i] = 0;
+  for (int i = 0; i < TW_NAF_LEN; ++i) {
+    int di = 0;
+    if (k & 1) {
+      di = (int)(k & 7);
+      if (di >= 4) di -= 8;
+      if (di > 0) k -= (unsigned)di;
+      else k += (unsigned)(-di);
+    }
+    words[i >> 3] |= ((uint32_t)di & 15u) << (4 * (i & 7));
+    if (di) top = i;
+    k >>= 1;
+  }
+  return top;
+}
+
+static tw_naf twiddle_naf(const bteh::Fr& k) {
+  uint32_t dec[16];
+  twiddle_decomposition(k, dec);
+  tw_naf t;
+  memset(&t, 0, sizeof(t));
+  for (int g = 0; g < 2; ++g) {
+    int top = -1;
+    for (int c = 0; c < 2; ++c) {
+      unsigned __int128 v = 0;
+      for (int i = 3; i >= 0; --i) v = (v << 32) | dec[8 * g + 4 * c + i];
+      top = std::max(top, wnaf3(v, t.d[g][c]));
+    }
+    t.top[g] = (int16_t)top;
+  }
+  return t;
+}
+
 struct DomainTables {
   uint32_t* dec = nullptr;       // n/2 twiddle decompositions (16 x u32 each)
+  tw_naf* naf = nullptr;         // the same as width-3 NAF digits
   uint32_t* twiddles = nullptr;  // n/2 canonical scalars (8 x u32 each): w^j
   cyc_plan* plans = nullptr;     // n/2 plans of w^j
   uint32_t* size_inv = nullptr;  // n^-1 (8 x u32)
@@ -156,6 +200,10 @@ static int domain_tables(int logn, bool inverse, bool want_plans, DomainTables**
     for (size_t j = 0; j < tw.size(); ++j) twiddle_decomposition(tw[j], &dec[16 * j]);
     BTE_HIP(hipMalloc((void**)&t.dec, dec.size() * 4));
     BTE_HIP(hipMemcpy(t.dec, dec.data(), dec.size() * 4, hipMemcpyHostToDevice));
+    std::vector<tw_naf> naf(tw.size());
+    for (size_t j = 0; j < tw.size(); ++j) naf[j] = twiddle_naf(tw[j]);
+    BTE_HIP(hipMalloc((void**)&t.naf, naf.size() * sizeof(tw_naf)));
+    BTE_HIP(hipMemcpy(t.naf, naf.data(), naf.size() * sizeof(tw_naf), hipMemcpyHostToDevice));
     BTE_HIP(hipMalloc((void**)&t.size_inv, 32));
     BTE_HIP(hipMemcpy(t.size_inv, ninv.l, 32, hipMemcpyHostToDevice));
   }
@@ -220,9 +268,9 @@ BTE_DEV size_t butterfly_slot(size_t g, size_t n, int log_half, size_t count, si
 }
 
 // One DIT stage for butterfly size 2 * 2^log_half; twiddle index k << (logn - 1 - log_half).
-template <class F>
+template <class F, class TW>
 __global__ void __launch_bounds__(64) k_group_stage(jac<F>* __restrict__ v, size_t n, int logn, int log_half,
-                                                    const uint32_t* __restrict__ dec, size_t count) {
+                                                    const TW* __restrict__ tw, size_t count) {
   const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (g >= (n / 2) * count) return;
   const size_t half = (size_t)1 << log_half;
@@ -230,7 +278,7 @@ __global__ void __launch_bounds__(64) k_group_stage(jac<F>* __restrict__ v, size
   const size_t i = butterfly_slot(g, n, log_half, count, k);
   const jac<F> u = v[i];
   jac<F> w = v[i + half];
-  if (k) w = twiddle_mul(w, dec + 16 * (k << (logn - 1 - log_half)));
+  if (k) w = twiddle_mul(w, twiddle_at(tw, k << (logn - 1 - log_half)));
   v[i] = jac_add(u, w);
   v[i + half] = jac_sub(u, w);
 }
@@ -354,9 +402,14 @@ static int group_fft(jac<F>* v, size_t n, size_t count, bool inverse, bool scale
   pt.begin();
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse<jac<F>>), dim3(blocks_for(n * count, 256)), dim3(256), 0, s, v, n,
                      logn, count);
-  for (int lh = 0; lh < logn; ++lh)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage<F>), dim3(blocks_for(n / 2 * count, 64)), dim3(64), 0, s, v, n,
-                       logn, lh, (const uint32_t*)t->dec, count);
+  for (int lh = 0; lh < logn; ++lh) {
+    if (naf_stage(n / 2 * count, lh))
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage<F, tw_naf>), dim3(blocks_for(n / 2 * count, 64)), dim3(64), 0,
+                         s, v, n, logn, lh, (const tw_naf*)t->naf, count);
+    else
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage<F, uint32_t>), dim3(blocks_for(n / 2 * count, 64)), dim3(64),
+                         0, s, v, n, logn, lh, (const uint32_t*)t->dec, count);
+  }
   pt.mark("stages");
   if (inverse && scaled)
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_scale<F>), dim3(blocks_for(n * count, 64)), dim3(64), 0, s, v,
@@ -477,11 +530,11 @@ static void stage_order(const std::vector<uint64_t>& off, const std::vector<uint
     }
 }
 
-template <class F>
+template <class F, class TW>
 __global__ void __launch_bounds__(64) k_group_stage_ord(jac<F>* __restrict__ v, const uint32_t* __restrict__ order,
                                                         uint64_t nbf, const uint64_t* __restrict__ off,
                                                         uint32_t count, int logmax, int log_half,
-                                                        const uint32_t* __restrict__ dec) {
+                                                        const TW* __restrict__ tw) {
   const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   if (g >= nbf) return;
   const size_t i = order[g];
@@ -490,7 +543,7 @@ __global__ void __launch_bounds__(64) k_group_stage_ord(jac<F>* __restrict__ v, 
   const size_t k = (i - off[f]) & (half - 1);
   const jac<F> u = v[i];
   jac<F> w = v[i + half];
-  if (k) w = twiddle_mul(w, dec + 16 * (k << (logmax - 1 - log_half)));
+  if (k) w = twiddle_mul(w, twiddle_at(tw, k << (logmax - 1 - log_half)));
   v[i] = jac_add(u, w);
   v[i + half] = jac_sub(u, w);
 }
@@ -550,9 +603,13 @@ static int group_fft_segmented(jac<F>* v, const int* log_sizes, size_t count, bo
                      (const uint64_t*)d_off, (const uint8_t*)d_logs, (uint32_t)count, total);
   for (int lh = 0; lh < logmax; ++lh) {
     const uint64_t nbf = start[lh + 1] - start[lh];
-    if (nbf)
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage_ord<F>), dim3(blocks_for(nbf, 64)), dim3(64), 0, s, v,
+    if (nbf && naf_stage(nbf, lh))
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage_ord<F, tw_naf>), dim3(blocks_for(nbf, 64)), dim3(64), 0, s, v,
                          (const uint32_t*)d_order + start[lh], nbf, (const uint64_t*)d_off, (uint32_t)count, logmax,
+                         lh, (const tw_naf*)t->naf);
+    else if (nbf)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_stage_ord<F, uint32_t>), dim3(blocks_for(nbf, 64)), dim3(64), 0, s,
+                         v, (const uint32_t*)d_order + start[lh], nbf, (const uint64_t*)d_off, (uint32_t)count, logmax,
                          lh, (const uint32_t*)t->dec);
   }
   if (inverse && scaled)
