@@ -1,6 +1,7 @@
 // BLS12-381 base field Fp on gfx950: 12 x 32-bit little-endian limbs in Montgomery form with
 // R = 2^384.  That is bit-for-bit the blst_fp layout (6 x u64 LE, R = 2^384), so values cross the
-// C ABI without conversion.  Every function keeps values fully reduced to [0, p).
+// C ABI without conversion.  Every function on fp keeps values fully reduced to [0, p), except
+// fp_add_raw, whose sum below 2p feeds only the double-width products of lazy reduction.
 //
 // Multiplication is Montgomery with product scanning on v_mad_u64_u32 carry chains.  Because
 // p < 2^381 the unreduced product is below 2p < 2^384, so one conditional subtraction finishes it.
