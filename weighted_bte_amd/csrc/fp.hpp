@@ -228,6 +228,24 @@ BTE_DEV fpw fpw_sub(const fpw& a, const fpw& b) {
   return r;
 }
 
+// k p^2 - b for k in {1, 2, 4} and b < k p^2.
+BTE_DEV fpw fpw_kp2_sub(int k, const fpw& b) {
+  constexpr uint32_t P2[24] = {BTE_P2_LIMBS};
+  const int sh = k == 4 ? 2 : k == 2 ? 1 : 0;
+  fpw kp;
+  uint32_t prev = 0;
+#pragma unroll
+  for (int i = 0; i < 24; ++i) {  // k p^2 < 2^765: the shift loses no bits
+    kp.l[i] = sh ? (P2[i] << sh) | (prev >> (32 - sh)) : P2[i];
+    prev = P2[i];
+  }
+  fpw r;
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 24; ++i) r.l[i] = __builtin_subc(kp.l[i], b.l[i], borrow, &borrow);
+  return r;
+}
+
 // p^2 - b for b < p^2.
 BTE_DEV fpw fpw_p2_sub(const fpw& b) {
   constexpr uint32_t P2[24] = {BTE_P2_LIMBS};

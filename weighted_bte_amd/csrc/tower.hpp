@@ -204,11 +204,25 @@ BTE_NOINL fp12 fp12_frobenius(const fp12& a, int k) {
 }
 
 // (a + b s)^2 in Fp4 = Fp2[s]/(s^2 - xi).
+// With lazy reduction: each Fp2 square x^2 = (x0 + x1)(x0 - x1) + 2 x0 x1 u is kept at double
+// width (operands reduced, so the parts are below p^2 and 2 p^2), c0 = xi t1 + t0 and
+// c1 = (a + b)^2 - t0 - t1 are formed there with multiples of p^2 keeping every value
+// non-negative and below 6 p^2 < p 2^384, and four Montgomery reductions replace six.
 BTE_DEV void fp4_sqr(fp2& c0, fp2& c1, const fp2& a, const fp2& b) {
-  fp2 t0 = fp2_sqr(a);
-  fp2 t1 = fp2_sqr(b);
-  c0 = fp2_add(fp2_mul_xi(t1), t0);
-  c1 = fp2_sub(fp2_sub(fp2_sqr(fp2_add(a, b)), t0), t1);
+  const fpw A0 = fp_mul_wide(fp_add(a.c0, a.c1), fp_sub(a.c0, a.c1));
+  const fpw a01 = fp_mul_wide(a.c0, a.c1);
+  const fpw A1 = fpw_add(a01, a01);
+  const fpw B0 = fp_mul_wide(fp_add(b.c0, b.c1), fp_sub(b.c0, b.c1));
+  const fpw b01 = fp_mul_wide(b.c0, b.c1);
+  const fpw B1 = fpw_add(b01, b01);
+  c0.c0 = fp_redc_wide(fpw_add(fpw_add(A0, B0), fpw_kp2_sub(2, B1)));  // B0 - B1 + A0
+  c0.c1 = fp_redc_wide(fpw_add(fpw_add(A1, B0), B1));                 // B0 + B1 + A1
+  const fpw P0 = fpw_add(A0, B0), P1 = fpw_add(A1, B1);               // < 2 p^2, < 4 p^2
+  const fp2 s = fp2_add(a, b);
+  const fpw S0 = fp_mul_wide(fp_add(s.c0, s.c1), fp_sub(s.c0, s.c1));
+  const fpw s01 = fp_mul_wide(s.c0, s.c1);
+  c1.c0 = fp_redc_wide(fpw_add(S0, fpw_kp2_sub(2, P0)));
+  c1.c1 = fp_redc_wide(fpw_add(fpw_add(s01, s01), fpw_kp2_sub(4, P1)));
 }
 
 // Granger-Scott squaring, valid on the cyclotomic subgroup.  With f = A + B w + C w^2 over
