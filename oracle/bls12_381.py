@@ -21,9 +21,12 @@ compressed encodings), closed forms and invariants:
                                    (weighted_btx/src/fft.rs:660-681)
   Decomposed GT scalar action .. pinned: equals Gt^s on r-torsion inputs
                                    (weighted_btx/src/fft.rs:811-846)
-  GT absolute value ............ parity pinned only up to the Miller-loop sign convention for
-                                   negative x (conjugate at the end), which blst's un-vendored
-                                   source fixes; no reference test holds a GT value.
+  GT absolute value ............ pinned: e^c == t(Q,P)^(3L) against a reduced Tate pairing from
+                                   an independent Miller loop over r (Hess-Smart-Vercauteren
+                                   ate relation with T = x < 0; tests/test_oracle_gt_sign.py),
+                                   which fixes the conjugation for negative x.  Still unpinned:
+                                   that blst's un-vendored miller_loop_n makes the same (textbook)
+                                   choice; no reference test holds a GT value.
 """
 
 from __future__ import annotations
