@@ -310,3 +310,11 @@ def test_share_verification_each_and_batched(api):
     tampered[2 * 144:3 * 144] = api.to_device(O.g1_jac_to_bytes(bad_pt))
     assert api.btx_verify_shares(tampered, parties, vk, len(weights), L, ct1, valid) == [True, True, False, True]
     assert not api.btx_verify_shares_batched(tampered, parties, rho, vk, len(weights), L, ct1, valid)
+    # a zero challenge would drop the tampered share from the folded check: refused, as is a
+    # non-canonical one (decryption.rs:363-370 draws nonzero challenges)
+    from weighted_bte_amd import BteError
+    for bad in (0, O.R):
+        rho_bad = list(rho)
+        rho_bad[2] = bad
+        with pytest.raises(BteError):
+            api.btx_verify_shares_batched(tampered, parties, rho_bad, vk, len(weights), L, ct1, valid)

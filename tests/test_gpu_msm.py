@@ -231,6 +231,44 @@ def test_batched_msm_uneven_segments(api, group):
             assert read(out[width * sgm: width * (sgm + 1)]) == (mul(want) if want % O.R else None)
 
 
+@pytest.mark.parametrize("group", [1, 2])
+def test_batched_msm_reports_out_of_range_index(api, group):
+    """An index at or beyond n_points in a batched, gathered MSM is skipped on the device and
+    reported through the status word (the wrapper raises); a valid call reports BTE_OK."""
+    import ctypes
+
+    from weighted_bte_amd import call
+
+    rnd = random.Random(700 + group)
+    mul = g1_mul_gen if group == 1 else g2_mul_gen
+    pack = g1_points_bytes if group == 1 else g2_points_bytes
+    fn = api.g1_multi_exp_batched if group == 1 else api.g2_multi_exp_batched
+    ks, pts = _rand_bases(rnd, 8, mul)
+    sc = [rnd.randrange(O.R) for _ in range(10)]
+    offs = [0, 4, 10]
+    good = [rnd.randrange(8) for _ in range(10)]
+    fn(api.to_device(pack(pts)), api.to_device(pack_scalars(sc)), offs,
+       torch.tensor(good, dtype=torch.int32, device="cuda"))
+    for bad_at, bad in ((2, 8), (9, 1 << 20)):
+        idx = list(good)
+        idx[bad_at] = bad
+        with pytest.raises(ValueError):
+            fn(api.to_device(pack(pts)), api.to_device(pack_scalars(sc)), offs,
+               torch.tensor(idx, dtype=torch.int32, device="cuda"))
+    # through the C ABI directly: status is BTE_EINVAL (1)
+    name = "bte_g1_msm_batched" if group == 1 else "bte_g2_msm_batched"
+    P = api.to_device(pack(pts))
+    S = api.to_device(pack_scalars(sc))
+    I = torch.tensor(good[:9] + [8], dtype=torch.int32, device="cuda")
+    off = torch.tensor(offs, dtype=torch.int64, device="cuda")
+    out = torch.empty(2 * (144 if group == 1 else 288), dtype=torch.uint8, device="cuda")
+    status = torch.full((1,), -1, dtype=torch.int32, device="cuda")
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+    call(name, ptr(P), 8, ptr(I), ptr(S), 10, ptr(off), 2, ptr(out), ptr(status),
+         ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert int(status.item()) == 1
+
+
 def test_g2_msm_closed_form_2_16(api):
     """Config-2 G2 at 2^16 (bases k_i * G2, 10% sparse scalars), compressed-byte equality."""
     n = 1 << 16

@@ -75,3 +75,25 @@ def test_decrypt_requests_two_in_flight(dev):
     assert srv.launches == 3
     for r, want in zip(reqs, expected):
         assert api.to_host(r.messages) == want
+
+
+def test_serving_accepts_any_batch_up_to_max(dev):
+    """validate_batch_size (weighted_btx/src/decryption.rs:711-722) accepts any 1 <= B <= L: with
+    L = 100 a request of B = 100 runs against the state prepared for L (same FFT size as 128);
+    B = 101 is refused."""
+    from weighted_bte_amd import api, serving, workloads
+
+    weights = [1, 2, 1, 3]
+    wl = workloads.BtxWorkload(weights, threshold=3, max_batch=100, seed=19)
+    com = serving.Committee(wl.material, wl.W, wl.L, wl.offsets, workloads.heaviest_first(weights, wl.t))
+    reqs, expected = make_requests(api, workloads, serving, [(wl, com.accepted)], [(0, 100), (0, 70)])
+    srv = serving.BtxServer([com])
+    for r in reqs:
+        srv.submit(r)
+    assert [r.padded for r in reqs] == [100, 100]
+    srv.drain()
+    for r, want in zip(reqs, expected):
+        assert api.to_host(r.messages) == want
+        assert api.to_host(srv.decrypt_alone(r)) == want
+    with pytest.raises(ValueError):
+        srv.submit(serving.Request(99, 0, 101, reqs[0].ct1, reqs[0].ct2, reqs[0].shares))

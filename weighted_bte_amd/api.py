@@ -30,11 +30,16 @@ def _stream():
 
 
 def _dev(t):
+    """Checks a device input and returns it unchanged.  Non-contiguous tensors are refused rather
+    than copied: a temporary copy would be released (and its memory handed out again by the
+    caching allocator) while the asynchronous kernels still read it."""
     if not t.is_cuda:
         raise ValueError("expected a device tensor")
     if t.dtype != torch.uint8 and t.dtype != torch.int32:
         raise ValueError("expected uint8 or int32 tensor")
-    return t.contiguous()
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return t
 
 
 def to_device(data: bytes, device="cuda"):
@@ -168,8 +173,11 @@ def _msm_batched(fn, width, jwidth, points, scalar_bytes, seg_offsets, indices):
     offs = torch.tensor(seg_offsets, dtype=torch.int64, device=points.device)
     nseg = len(seg_offsets) - 1
     out = _empty(nseg * jwidth, points)
+    status = torch.zeros(1, dtype=torch.int32, device=points.device)
     call(fn, _ptr(points), points.numel() // width, _ptr(indices), _ptr(scalar_bytes), nterms,
-         _ptr(offs), nseg, _ptr(out), _stream())
+         _ptr(offs), nseg, _ptr(out), _ptr(status), _stream())
+    if indices is not None and int(status.item()) != 0:  # waits for the stream
+        raise ValueError("point index out of range")
     return out
 
 
@@ -261,6 +269,17 @@ def fft_g1_segmented(values, log_sizes, inverse=False, scaled=True):
     arr = (ctypes.c_int * len(log_sizes))(*log_sizes)
     call("bte_fft_g1_segmented", _ptr(v), arr, len(log_sizes), 1 if inverse else 0, 1 if scaled else 0,
          _stream())
+    return v
+
+
+def fft_fr(values, inverse=False, scaled=True, count=1):
+    """Radix2Domain::{fft, ifft} over Scalar (fft.rs:197-234) on canonical 32-byte scalars
+    (device), `count` transforms of equal power-of-two size back to back."""
+    v = _dev(values).clone()
+    n = v.numel() // SCALAR // count
+    if n * count * SCALAR != v.numel():
+        raise ValueError("values.len() != count * n")
+    call("bte_fft_fr_batched", _ptr(v), n, count, 1 if inverse else 0, 1 if scaled else 0, _stream())
     return v
 
 
@@ -381,7 +400,8 @@ def full_final_exponentiation(values):
 
 # --- a16: weighted_btx phases (weighted_btx/src/decryption.rs) ------------------------------
 def lagrange_at_zero(total_weight, selected):
-    """interpolation.rs:31-52 (host restatement in the library); returns canonical ints."""
+    """interpolation.rs:31-52 (direct formula up to 64 points, device product tree above);
+    returns canonical ints."""
     sel = (ctypes.c_uint64 * len(selected))(*selected)
     out = ctypes.create_string_buffer(32 * len(selected))
     call("bte_lagrange_at_zero", total_weight, sel, len(selected), out)

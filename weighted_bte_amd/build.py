@@ -46,11 +46,26 @@ def compile_one(src):
     return obj
 
 
+def includes(path, seen=None):
+    """The file and every local header it includes, transitively (#include "...")."""
+    seen = set() if seen is None else seen
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    with open(path) as f:
+        for line in f:
+            t = line.strip()
+            if t.startswith("#include \""):
+                inc = t.split('"')[1]
+                includes(os.path.normpath(os.path.join(os.path.dirname(path), inc)), seen)
+    return seen
+
+
 def up_to_date(src):
     obj = os.path.join(BUILD, src[:-4] + ".o")
     if not os.path.exists(obj):
         return False
-    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+    deps = includes(os.path.join(CSRC, src))
     return all(os.path.getmtime(d) <= os.path.getmtime(obj) for d in deps)
 
 

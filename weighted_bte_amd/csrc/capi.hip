@@ -2,6 +2,7 @@
 #include <stdarg.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 
@@ -18,6 +19,17 @@ int set_error(int code, const char* fmt, ...) {
   vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
   va_end(ap);
   return code;
+}
+
+// Test support: the g_fail_at-th allocation from now made by the library's table caches fails as
+// if the device were out of memory (bte_test_fail_alloc); 0 means none.
+static std::atomic<int> g_fail_at{0};
+
+bool test_alloc_should_fail() {
+  int v = g_fail_at.load();
+  while (v > 0)
+    if (g_fail_at.compare_exchange_weak(v, v - 1)) return v == 1;
+  return false;
 }
 
 static bool g_profile_on = false;
@@ -122,6 +134,12 @@ extern "C" int bte_probe_mac_rate(double* macs_per_second, void* stream) {
 }
 
 extern "C" const char* bte_last_error(void) { return g_last_error; }
+
+extern "C" int bte_test_fail_alloc(int nth) {
+  if (nth < 0) return set_error(BTE_EINVAL, "test_fail_alloc: negative count");
+  g_fail_at.store(nth);
+  return BTE_OK;
+}
 
 extern "C" const char* bte_version(void) { return "weighted_bte_amd 0.1 gfx950"; }
 

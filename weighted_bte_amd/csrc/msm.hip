@@ -865,11 +865,22 @@ static int batched_window(size_t nterms, size_t nseg) {
   return std::max(3, std::min(12, c));
 }
 
+// status <- BTE_EINVAL when an index was out of range (the term was skipped), else BTE_OK.
+__global__ void k_msm_status(const int* __restrict__ err, int* __restrict__ status) {
+  if (threadIdx.x == 0) *status = *err ? BTE_EINVAL : BTE_OK;
+}
+
+// `status` (device int or null) receives BTE_EINVAL in stream order when some point index was
+// out of range; the library's own callers build their index arrays on the host in range.
 template <class F>
 static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t* index, const uint8_t* scalars,
                            const uint32_t* scalar_index, size_t nterms, const uint64_t* seg_offsets, size_t nseg, int c_force, jac<F>* out,
-                           hipStream_t s) {
-  if (nseg == 0) return BTE_OK;
+                           int* status, hipStream_t s) {
+  if (nseg == 0) {
+    if (status) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)status, (size_t)1, 0u);
+    BTE_CHECK_LAUNCH();
+    return BTE_OK;
+  }
   if (!seg_offsets || !out || (nterms && (!points || !scalars)))
     return set_error(BTE_EINVAL, "msm_batched: null pointer");
   if (((uintptr_t)scalars & 3) != 0) return set_error(BTE_EINVAL, "msm_batched: scalars must be 4-byte aligned");
@@ -948,6 +959,7 @@ static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t
                      (const jac<F>*)seg_acc, (const jac<F>*)seg_run, nsw, logL, logS, win);
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_combine<F>), dim3(blocks_for(nseg, 64)), dim3(64), 0, s,
                      (const jac<F>*)win, (uint32_t)nseg, nw, c, out);
+  if (status) hipLaunchKernelGGL(k_msm_status, dim3(1), dim3(64), 0, s, (const int*)err, status);
   BTE_CHECK_LAUNCH();
   // The workspace is reused by the next call on this stream; stream order protects it.
   return BTE_OK;
@@ -992,16 +1004,16 @@ extern "C" int bte_msm_set_window(int g1_or_g2, size_t n, int window_bits) {
 
 extern "C" int bte_g1_msm_batched(const bte_p1_affine* points, size_t n_points, const uint32_t* indices,
                                   const uint8_t* scalars, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
-                                  bte_p1* out, void* stream) {
+                                  bte_p1* out, int* status, void* stream) {
   return msm_batched_run<fp>((const aff<fp>*)points, n_points, indices, scalars, nullptr, nterms, seg_offsets, nseg, 0,
-                             (jac<fp>*)out, (hipStream_t)stream);
+                             (jac<fp>*)out, status, (hipStream_t)stream);
 }
 
 extern "C" int bte_g2_msm_batched(const bte_p2_affine* points, size_t n_points, const uint32_t* indices,
                                   const uint8_t* scalars, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
-                                  bte_p2* out, void* stream) {
+                                  bte_p2* out, int* status, void* stream) {
   return msm_batched_run<fp2>((const aff<fp2>*)points, n_points, indices, scalars, nullptr, nterms, seg_offsets, nseg, 0,
-                              (jac<fp2>*)out, (hipStream_t)stream);
+                              (jac<fp2>*)out, status, (hipStream_t)stream);
 }
 
 // Batched MSMs whose terms gather both the point (points[point_index[t]]) and the scalar
@@ -1012,12 +1024,12 @@ int g1_msm_batched_gather(const void* points, size_t n_points, const uint32_t* p
                           const uint32_t* scalar_index, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
                           int c, void* out, hipStream_t s) {
   return msm_batched_run<fp>((const aff<fp>*)points, n_points, point_index, scalars, scalar_index, nterms,
-                             seg_offsets, nseg, c, (jac<fp>*)out, s);
+                             seg_offsets, nseg, c, (jac<fp>*)out, nullptr, s);
 }
 int g2_msm_batched_gather(const void* points, size_t n_points, const uint32_t* point_index, const uint8_t* scalars,
                           const uint32_t* scalar_index, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
                           int c, void* out, hipStream_t s) {
   return msm_batched_run<fp2>((const aff<fp2>*)points, n_points, point_index, scalars, scalar_index, nterms,
-                              seg_offsets, nseg, c, (jac<fp2>*)out, s);
+                              seg_offsets, nseg, c, (jac<fp2>*)out, nullptr, s);
 }
 }  // namespace bte

@@ -177,6 +177,15 @@ extern "C" int bte_btx_verify_shares(const bte_p1* shares, const uint32_t* party
   return BTE_OK;
 }
 
+static bool canonical_fr(const uint8_t* b, bteh::Fr* out) {
+  memcpy(out->l, b, 32);
+  for (int i = 3; i >= 0; --i) {
+    if (out->l[i] < bteh::FR_R[i]) return true;
+    if (out->l[i] > bteh::FR_R[i]) return false;
+  }
+  return false;
+}
+
 extern "C" int bte_btx_verify_shares_batched(const bte_p1* shares, const uint32_t* party_index,
                                              const uint8_t* challenges, size_t n_shares,
                                              const bte_p2_affine* verification_keys, size_t n_parties,
@@ -189,6 +198,13 @@ extern "C" int bte_btx_verify_shares_batched(const bte_p1* shares, const uint32_
   if (T == 0) return BTE_OK;
   if (!shares || !party_index || !challenges || !verification_keys || !ct1 || !valid)
     return set_error(BTE_EINVAL, "verify_shares_batched: null pointer");
+  // The folded check is sound only for nonzero challenges (decryption.rs:363-370 draws them
+  // nonzero): a zero challenge would drop that party's share from both sides.
+  for (size_t j = 0; j < T; ++j) {
+    bteh::Fr c;
+    if (!canonical_fr(challenges + 32 * j, &c) || (c.l[0] | c.l[1] | c.l[2] | c.l[3]) == 0)
+      return set_error(BTE_EINVAL, "verify_shares_batched: challenge %zu is zero or not canonical", j);
+  }
   std::vector<uint32_t> slots;
   V_TRY(check_args(party_index, T, N, max_batch, batch_size, valid, slots));
   const size_t nv = slots.size();
@@ -248,15 +264,6 @@ extern "C" int bte_btx_verify_shares_batched(const bte_p1* shares, const uint32_
   BTE_CHECK_LAUNCH();
   *result = ok;
   return BTE_OK;
-}
-
-static bool canonical_fr(const uint8_t* b, bteh::Fr* out) {
-  memcpy(out->l, b, 32);
-  for (int i = 3; i >= 0; --i) {
-    if (out->l[i] < bteh::FR_R[i]) return true;
-    if (out->l[i] > bteh::FR_R[i]) return false;
-  }
-  return false;
 }
 
 extern "C" int bte_btx_verify_proofs(const bte_p1_affine* commitments, const bte_p1_affine* ct1,

@@ -6,7 +6,7 @@ keys and the opening keys' line tables, per padded batch size) resident in HBM. 
 budget and a launch slot are free, it admits the waiting requests that fit and runs all of them,
 whatever their committees and batch sizes, through one fused precompute + open
 (bte_btx_decrypt_requests): every stage is one launch over every admitted request.  A request
-of B ciphertexts runs against the state prepared for the next power of two, the extra slots
+of B ciphertexts runs against the state prepared for min(nextpow2(B), L), the extra slots
 being empty (the reference's None).  Up to `max_inflight` such launches overlap on the device,
 each on its own stream; requests that arrive while others run are admitted as soon as budget
 and a slot free up, and completion is observed with stream events.
@@ -47,10 +47,11 @@ class Request:
     done: float = 0.0
     messages: torch.Tensor = None
     valid: list = None         # B flags (None: all valid)
+    padded: int = 0            # batch size of the prepared state it runs against (set by submit)
 
     def footprint(self):
         """HBM bytes held while in flight: padded inputs, Miller/FFT scratch and outputs."""
-        bp = next_pow2(self.batch)
+        bp = self.padded or next_pow2(self.batch)
         m = 2 * bp
         t = self.shares.numel() // G1J
         return bp * (G1A + 2 * F12) + m * (G1J + G1A + 3 * F12) + bp * t * (G1A + 192 + F12)
@@ -86,8 +87,13 @@ class BtxServer:
         self.launches = 0
 
     def submit(self, req: Request):
-        if next_pow2(req.batch) > self.committees[req.committee].max_batch:
+        """Queue a request.  Any 1 <= B <= L is accepted (validate_batch_size,
+        weighted_btx/src/decryption.rs:711-722); it runs against the state prepared for
+        min(nextpow2(B), L), whose FFT size nextpow2(2 B') is that of nextpow2(B)."""
+        L = self.committees[req.committee].max_batch
+        if req.batch < 1 or req.batch > L:
             raise ValueError("BatchTooLarge")
+        req.padded = min(next_pow2(req.batch), L)
         self.waiting.append(req)
 
     def _used(self):
@@ -111,7 +117,7 @@ class BtxServer:
         work = []
         for r in reqs:
             com = self.committees[r.committee]
-            bp = next_pow2(r.batch)
+            bp = r.padded
             lines, keys, key_lines = com.prepare(bp)
             work.append({"ct1": r.ct1, "ct2": r.ct2, "valid": r.valid or [True] * r.batch, "batch_size": bp,
                          "lines": lines, "keys": keys, "key_lines": key_lines, "shares": r.shares,
