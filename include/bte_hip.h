@@ -65,16 +65,19 @@ int bte_g2_msm_indexed(const bte_p2_affine* points, size_t n_points, const uint3
 
 /* Batched MSMs of uneven sizes in one asynchronous launch sequence (no host synchronisation,
  * graph-capturable): out[s] = sum over terms t in [seg_offsets[s], seg_offsets[s+1]) of
- * scalars[t] * P(t), P(t) = points[indices[t]] when `indices` is non-NULL (bounds-checked against
- * n_points) else points[t].  seg_offsets is a device array of nseg+1 u64 with
- * seg_offsets[nseg] = nterms.  Replaces the rayon fan-out of per-party / per-slot MSMs around
+ * scalars[t] * P(t), P(t) = points[indices[t]] when `indices` is non-NULL else points[t].
+ * seg_offsets is a device array of nseg+1 u64 with seg_offsets[nseg] = nterms.  Indices are
+ * checked against n_points on the device: a term with an index out of range is skipped and
+ * `status` (a DEVICE int, may be NULL) receives BTE_EINVAL in stream order, else BTE_OK; read it
+ * after the stream reaches the call (the call itself cannot know without synchronising).
+ * Replaces the rayon fan-out of per-party / per-slot MSMs around
  * weighted_btx/src/decryption.rs:205-233 (partial_decrypt) and :465-481 (prepare_decryption). */
 int bte_g1_msm_batched(const bte_p1_affine* points, size_t n_points, const uint32_t* indices,
                        const uint8_t* scalars, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
-                       bte_p1* out, void* stream);
+                       bte_p1* out, int* status, void* stream);
 int bte_g2_msm_batched(const bte_p2_affine* points, size_t n_points, const uint32_t* indices,
                        const uint8_t* scalars, size_t nterms, const uint64_t* seg_offsets, size_t nseg,
-                       bte_p2* out, void* stream);
+                       bte_p2* out, int* status, void* stream);
 
 /* Window width override for the Pippenger kernels (0 = the built-in table by size). */
 int bte_msm_set_window(int g1_or_g2, size_t n, int window_bits);
@@ -105,6 +108,12 @@ int bte_g2_decompress(const uint8_t* in, size_t n, bte_p2_affine* points, uint8_
  * w^-1; `scaled` multiplies by n^-1 (inverse only).  A size that is not a power of two returns
  * BTE_EINVAL (the reference's Error::InvalidBatchSize).
  * ---------------------------------------------------------------------------------------- */
+
+/* weighted_btx/src/fft.rs:197-234 Radix2Domain::{fft, ifft} over Scalar: `values` holds n
+ * canonical 32-byte little-endian scalars (DEVICE), transformed in place; the batched form runs
+ * `count` transforms stored back to back.  Used by the Lagrange product tree below. */
+int bte_fft_fr(uint8_t* values, size_t n, int inverse, int scaled, void* stream);
+int bte_fft_fr_batched(uint8_t* values, size_t n, size_t count, int inverse, int scaled, void* stream);
 
 /* weighted_btx/src/fft.rs:197-234 Radix2Domain::{fft, ifft, ifft_unscaled} over G1Projective */
 int bte_fft_g1(bte_p1* values, size_t n, int inverse, int scaled, void* stream);
@@ -192,8 +201,14 @@ int bte_final_exp(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
  * ---------------------------------------------------------------------------------------- */
 
 /* weighted_btx/src/interpolation.rs:31-52 lagrange_at_zero over the first W points of the
- * radix-2 domain of size nextpow2(W); selected (n indices) and out (n x 32 bytes) are HOST. */
+ * radix-2 domain of size nextpow2(W); selected (n indices) and out (n x 32 bytes) are HOST.
+ * Above 64 points the coefficients come from the fast path of interpolation.rs:101-197 on the
+ * device (product tree of Fr transforms, then P' on the whole domain); a repeated or
+ * out-of-range index is BTE_EINVAL (Error::InvalidInterpolationSet). */
 int bte_lagrange_at_zero(size_t total_weight, const uint64_t* selected, size_t n, uint8_t* out);
+/* The same with `out` in DEVICE memory, queued on `stream` (selected stays HOST). */
+int bte_lagrange_at_zero_device(size_t total_weight, const uint64_t* selected, size_t n, uint8_t* out,
+                                void* stream);
 
 /* weighted_btx/src/decryption.rs:427-551 prepare_decryption.  party_offsets (N+1) and accepted
  * (T party indices, committee order) are HOST.  Outputs (device): kernel_lines, nextpow2(2B)
@@ -284,8 +299,9 @@ int bte_btx_verify_shares(const bte_p1* shares, const uint32_t* party_index, siz
                           const bte_p1_affine* ct1, const uint8_t* valid, size_t batch_size, uint8_t* ok_out,
                           void* stream);
 /* decryption.rs:356-422 verify_decryption_shares_batched: the shares folded with caller-chosen
- * nonzero challenges (T canonical scalars, HOST): one G1 MSM, one batched indexed G2 MSM over
- * the valid slots and one multi-pairing; *result (HOST) = 1 when the folded check holds. */
+ * nonzero challenges (T canonical scalars, HOST; a zero or non-canonical challenge is
+ * BTE_EINVAL): one G1 MSM, one batched indexed G2 MSM over the valid slots and one
+ * multi-pairing; *result (HOST) = 1 when the folded check holds. */
 int bte_btx_verify_shares_batched(const bte_p1* shares, const uint32_t* party_index, const uint8_t* challenges,
                                   size_t n_shares, const bte_p2_affine* verification_keys, size_t n_parties,
                                   size_t max_batch, const bte_p1_affine* ct1, const uint8_t* valid,
@@ -392,6 +408,11 @@ int bte_profile_enable(int on);
 int bte_profile_read(char* buf, size_t len);
 /* Measured 32x32->64 multiply-accumulates per second of the whole device. */
 int bte_probe_mac_rate(double* macs_per_second, void* stream);
+
+/* Test support (no counterpart in the reference): the nth allocation from now of the library's
+ * cached tables (FFT domain twiddles and plans) fails as if the device were out of memory (0
+ * cancels), so a test can check that a failed build leaves nothing half-made behind. */
+int bte_test_fail_alloc(int nth);
 
 /* ---------------------------------------------------------------------------------------- *
  * Device memory helpers for hosts without a HIP binding (the Rust FFI crate).

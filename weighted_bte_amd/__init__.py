@@ -32,8 +32,8 @@ _SIGNATURES = {
     "bte_g2_msm": [_vp, _vp, _sz, _vp, _vp],
     "bte_g2_msm_indexed": [_vp, _sz, _vp, _vp, _sz, _vp, _vp],
     "bte_msm_set_window": [_i, _sz, _i],
-    "bte_g1_msm_batched": [_vp, _sz, _vp, _vp, _sz, _vp, _sz, _vp, _vp],
-    "bte_g2_msm_batched": [_vp, _sz, _vp, _vp, _sz, _vp, _sz, _vp, _vp],
+    "bte_g1_msm_batched": [_vp, _sz, _vp, _vp, _sz, _vp, _sz, _vp, _vp, _vp],
+    "bte_g2_msm_batched": [_vp, _sz, _vp, _vp, _sz, _vp, _sz, _vp, _vp, _vp],
     "bte_g1_batch_normalize": [_vp, _sz, _vp, _vp],
     "bte_g2_batch_normalize": [_vp, _sz, _vp, _vp],
     "bte_g1_generator_mul": [_vp, _sz, _vp, _vp],
@@ -53,6 +53,9 @@ _SIGNATURES = {
     "bte_final_exp_hard": [_vp, _sz, _vp, _vp],
     "bte_final_exp": [_vp, _sz, _vp, _vp],
     "bte_lagrange_at_zero": [_sz, _vp, _sz, _vp],
+    "bte_lagrange_at_zero_device": [_sz, _vp, _sz, _vp, _vp],
+    "bte_fft_fr": [_vp, _sz, _i, _i, _vp],
+    "bte_fft_fr_batched": [_vp, _sz, _sz, _i, _i, _vp],
     "bte_btx_prepare": [_vp, _sz, _sz, _vp, _sz, _vp, _sz, _sz, _vp, _vp, _vp],
     "bte_btx_precompute": [_vp, _vp, _sz, _vp, _vp, _vp],
     "bte_btx_open": [_vp, _sz, _vp, _vp, _vp, _vp, _sz, _vp, _vp],
@@ -99,6 +102,7 @@ _SIGNATURES = {
     "bte_memcpy_h2d": [_vp, _vp, _sz, _vp],
     "bte_memcpy_d2h": [_vp, _vp, _sz, _vp],
     "bte_stream_synchronize": [_vp],
+    "bte_test_fail_alloc": [_i],
 }
 _RESTYPES = {"bte_last_error": ctypes.c_char_p, "bte_version": ctypes.c_char_p}
 

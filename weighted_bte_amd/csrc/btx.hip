@@ -1,7 +1,7 @@
 // weighted_btx batch decryption on the GPU: the host control flow of
 // weighted_btx/src/decryption.rs (prepare_decryption :427-551, precompute_batch :555-603,
-// open_batch :607-695, partial_decrypt :205-233) over this library's kernels, plus the host
-// restatement of lagrange_at_zero (weighted_btx/src/interpolation.rs:31-140).  Protocol
+// open_batch :607-695, partial_decrypt :205-233) over this library's kernels; the Lagrange
+// coefficients (weighted_btx/src/interpolation.rs:31-140) come from fr_ntt.hip.  Protocol
 // bookkeeping (proof checks, digests, setup and committee binding) stays with the caller; these
 // entry points take the already-validated arrays.
 //
@@ -38,108 +38,6 @@ int bte_final_exp(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
 }
 
 namespace bte {
-
-// ---------------------------------------------------------------------------------------------
-// Host: Lagrange coefficients at zero (interpolation.rs:31-140).  The selected points are
-// x_v = w^v, w the generator of the radix-2 domain of size nextpow2(W).  The coefficients are
-// unique field values; this restatement uses L_v(0) = prod_{u != v} x_u / (x_u - x_v), with one
-// batched inversion, and the closed form 1/M for a complete domain (interpolation.rs:36-44).
-// ---------------------------------------------------------------------------------------------
-// den_v = x_v prod_{u != v} (x_u - x_v) (Montgomery in and out), one thread per point.
-__global__ void __launch_bounds__(256) k_lagrange_den(const fr* __restrict__ x, size_t n, fr* __restrict__ den) {
-  const size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  const fr xv = x[v];
-  fr d = xv;
-  for (size_t u = 0; u < n; ++u)
-    if (u != v) d = fr_mont_mul(d, fr_sub(x[u], xv));
-  den[v] = d;
-}
-
-__global__ void k_fr_mont(fr* v, size_t n, int to) {
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i < n) v[i] = to ? fr_to_mont(v[i]) : fr_from_mont(v[i]);
-}
-
-static constexpr size_t LAGRANGE_GPU_MIN = 512;
-
-// Synchronous (legacy default stream): the host interface of lagrange_at_zero has no stream.
-static int lagrange_denominators_gpu(const std::vector<bteh::Fr>& x, std::vector<bteh::Fr>& den) {
-  const size_t n = x.size();
-  fr *d_x = nullptr, *d_den = nullptr;
-  BTE_HIP(hipMalloc((void**)&d_x, n * sizeof(fr)));
-  hipError_t e = hipMalloc((void**)&d_den, n * sizeof(fr));
-  if (e == hipSuccess) e = hipMemcpy(d_x, x.data(), n * sizeof(fr), hipMemcpyHostToDevice);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_fr_mont, dim3(blocks_for(n, 256)), dim3(256), 0, 0, d_x, n, 1);
-    hipLaunchKernelGGL(k_lagrange_den, dim3(blocks_for(n, 256)), dim3(256), 0, 0, (const fr*)d_x, n, d_den);
-    hipLaunchKernelGGL(k_fr_mont, dim3(blocks_for(n, 256)), dim3(256), 0, 0, d_den, n, 0);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipMemcpy(den.data(), d_den, n * sizeof(fr), hipMemcpyDeviceToHost);
-  (void)hipFree(d_x);
-  if (d_den) (void)hipFree(d_den);
-  if (e != hipSuccess) return set_error(BTE_EDEVICE, "lagrange_at_zero: %s", hipGetErrorString(e));
-  return BTE_OK;
-}
-
-static int lagrange_at_zero(size_t total_weight, const std::vector<uint64_t>& sel, std::vector<bteh::Fr>& out) {
-  using namespace bteh;
-  const size_t n = sel.size();
-  if (total_weight == 0 || n == 0) return set_error(BTE_EINVAL, "InvalidInterpolationSet");
-  std::vector<char> seen(total_weight, 0);
-  for (uint64_t v : sel) {
-    if (v >= total_weight || seen[v]) return set_error(BTE_EINVAL, "InvalidInterpolationSet");
-    seen[v] = 1;
-  }
-  size_t dom = 1;
-  int logd = 0;
-  while (dom < total_weight) {
-    dom <<= 1;
-    ++logd;
-  }
-  out.assign(n, Fr{});
-  if (n == total_weight && dom == total_weight) {
-    Fr inv = fr_inv(fr_from_u64(total_weight));
-    for (auto& c : out) c = inv;
-    return BTE_OK;
-  }
-  Fr w;
-  memcpy(w.l, FR_ROOT_OF_UNITY, 32);
-  for (int i = logd; i < FR_S; ++i) w = fr_mul(w, w);
-  std::vector<Fr> x(n);
-  for (size_t i = 0; i < n; ++i) x[i] = fr_pow_u64(w, sel[i]);
-  // num_v = prod_{u != v} x_u ; den_v = prod_{u != v} (x_u - x_v)
-  Fr prod_all = fr_from_u64(1);
-  for (size_t i = 0; i < n; ++i) prod_all = fr_mul(prod_all, x[i]);
-  std::vector<Fr> den(n);
-  if (n >= LAGRANGE_GPU_MIN) {
-    // the n^2 products on the device, one thread per point
-    int rc = lagrange_denominators_gpu(x, den);
-    if (rc) return rc;
-  } else {
-    for (size_t v = 0; v < n; ++v) {
-      Fr d = x[v];  // folds the 1/x_v of num_v = prod_all / x_v into the denominator
-      for (size_t u = 0; u < n; ++u)
-        if (u != v) d = fr_mul(d, fr_sub(x[u], x[v]));
-      den[v] = d;
-    }
-  }
-  // batch inversion of den
-  std::vector<Fr> pre(n);
-  Fr acc = fr_from_u64(1);
-  for (size_t i = 0; i < n; ++i) {
-    pre[i] = acc;
-    acc = fr_mul(acc, den[i]);
-  }
-  Fr inv = fr_inv(acc);
-  for (size_t i = n; i-- > 0;) {
-    Fr di = fr_mul(inv, pre[i]);
-    inv = fr_mul(inv, den[i]);
-    out[i] = fr_mul(prod_all, di);
-  }
-  return BTE_OK;
-}
 
 // ---------------------------------------------------------------------------------------------
 // Glue kernels
@@ -359,14 +257,20 @@ static size_t next_pow2(size_t v) {
 using namespace bte;
 
 // interpolation.rs:31-52 lagrange_at_zero over the first W points of the radix-2 domain of size
-// nextpow2(W); host memory: selected (n u64 indices), out (n canonical 32-byte scalars).
+// nextpow2(W); host memory: selected (n u64 indices), out (n canonical 32-byte scalars).  The
+// host interface has no stream: the device path (product tree, more than 64 points) runs on the
+// null stream and is waited for.
 extern "C" int bte_lagrange_at_zero(size_t total_weight, const uint64_t* selected, size_t n, uint8_t* out) {
   if (!selected || !out) return set_error(BTE_EINVAL, "lagrange_at_zero: null pointer");
-  std::vector<uint64_t> sel(selected, selected + n);
-  std::vector<bteh::Fr> c;
-  BTE_TRY(lagrange_at_zero(total_weight, sel, c));
-  memcpy(out, c.data(), 32 * n);
-  return BTE_OK;
+  uint8_t* d = nullptr;
+  if (n) BTE_HIP(hipMalloc((void**)&d, 32 * n));
+  int rc = lagrange_at_zero_device(total_weight, selected, n, nullptr, d, nullptr);
+  if (rc == BTE_OK) {
+    hipError_t e = hipMemcpy(out, d, 32 * n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = set_error(BTE_EDEVICE, "lagrange_at_zero: %s", hipGetErrorString(e));
+  }
+  if (d) (void)hipFree(d);
+  return rc;
 }
 
 // decryption.rs:427-551 prepare_decryption.  Host inputs: party_offsets (N+1), accepted party
@@ -394,8 +298,6 @@ extern "C" int bte_btx_prepare(const bte_p2_affine* material, size_t total_weigh
   }
   if (party_offsets[party_count] != W) return set_error(BTE_EINVAL, "btx_prepare: offsets do not sum to W");
   const size_t WT = sel.size();
-  std::vector<bteh::Fr> lam;
-  BTE_TRY(lagrange_at_zero(W, sel, lam));
   const size_t m = next_pow2(2 * B);
 
   // Negative MSMs: segment (slot, party p) over negative_power(slot+1)[range_p] with lambda.
@@ -432,7 +334,8 @@ extern "C" int bte_btx_prepare(const bte_p2_affine* material, size_t total_weigh
   g2j *opening, *negsum, *possum, *kernel;
   g2a* kernel_aff;
   uint32_t* d_minv;
-  BTE_TRY(sc.upload(&d_lam, (const uint8_t*)lam.data(), 32 * WT));
+  BTE_TRY(sc.get(&d_lam, 32 * WT));
+  BTE_TRY(lagrange_at_zero_device(W, sel.data(), WT, nullptr, d_lam, s));
   BTE_TRY(sc.upload(&d_nidx, nidx.data(), nidx.size()));
   BTE_TRY(sc.upload(&d_nsidx, nsidx.data(), nsidx.size()));
   BTE_TRY(sc.upload(&d_noff, noff.data(), noff.size()));

@@ -15,6 +15,9 @@ namespace bte {
 // Record the first failure for bte_last_error() and return its status code.
 int set_error(int code, const char* fmt, ...);
 
+// True when a failure was injected for the next cache allocation (bte_test_fail_next_allocs).
+bool test_alloc_should_fail();
+
 #define BTE_HIP(call)                                                                \
   do {                                                                               \
     hipError_t e_ = (call);                                                          \
@@ -24,6 +27,13 @@ int set_error(int code, const char* fmt, ...);
   } while (0)
 
 #define BTE_CHECK_LAUNCH() BTE_HIP(hipGetLastError())
+
+// Propagate a nonzero status from a library call.
+#define BTE_TRY_RC(x)      \
+  do {                     \
+    int rc_ = (x);         \
+    if (rc_) return rc_;   \
+  } while (0)
 
 // Optional per-phase timing with HIP events on the stream the kernels run on.  When enabled
 // (bte_profile_enable), drivers mark phase boundaries; after their stream sync the elapsed times
@@ -52,6 +62,12 @@ int g2_msm_batched_gather(const void* points, size_t n_points, const uint32_t* p
 // k = k1 + k2 lambda (lambda = |z|^2 - 1, the G1 endomorphism's eigenvalue) as k1[4] then k2[4]
 // little-endian u32 words, the layout g1_glv_mul reads (fft.hip).
 void glv_decompose_g1(const bteh::Fr& k, uint32_t out[8]);
+
+// Lagrange-at-zero coefficients (interpolation.rs:31-140) of the selected points (HOST indices)
+// of the size-nextpow2(W) radix-2 domain, as canonical scalars in DEVICE memory, each multiplied
+// by scale_canonical (32 bytes, HOST) when it is not null (fr_ntt.hip).
+int lagrange_at_zero_device(size_t W, const uint64_t* sel, size_t n, const uint8_t* scale_canonical, uint8_t* d_out,
+                            hipStream_t s);
 
 // Segmented cyclotomic prefix inverse transforms (fft.hip).
 struct fp12;

@@ -182,6 +182,26 @@ struct DomainTables {
 static std::mutex g_dom_mu;
 static std::map<std::tuple<int, int, int>, DomainTables> g_domains;  // (device, logn, inverse)
 
+// Device copy of a host array into a new allocation; on failure nothing stays allocated.
+template <class T> static hipError_t upload_new(T** dst, const T* src, size_t count) {
+  *dst = nullptr;
+  if (test_alloc_should_fail()) return hipErrorOutOfMemory;
+  hipError_t e = hipMalloc((void**)dst, count * sizeof(T));
+  if (e != hipSuccess) {
+    *dst = nullptr;
+    return e;
+  }
+  e = hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(*dst);
+    *dst = nullptr;
+  }
+  return e;
+}
+
+// Tables are built into locals and published in the cache only once every allocation and copy
+// has succeeded, so a failure (for example an allocation refused under memory pressure) leaves
+// no half-filled entry behind for later calls to read.
 static int domain_tables(int logn, bool inverse, bool want_plans, DomainTables** out, hipStream_t s) {
   int dev = 0;
   BTE_HIP(hipGetDevice(&dev));
@@ -194,18 +214,24 @@ static int domain_tables(int logn, bool inverse, bool want_plans, DomainTables**
     tw[0] = bteh::fr_from_u64(1);
     for (size_t j = 1; j < half; ++j) tw[j] = bteh::fr_mul(tw[j - 1], w);
     bteh::Fr ninv = bteh::fr_inv(bteh::fr_from_u64((uint64_t)1 << logn));
-    BTE_HIP(hipMalloc((void**)&t.twiddles, tw.size() * 32));
-    BTE_HIP(hipMemcpy(t.twiddles, tw.data(), tw.size() * 32, hipMemcpyHostToDevice));
     std::vector<uint32_t> dec(tw.size() * 16);
     for (size_t j = 0; j < tw.size(); ++j) twiddle_decomposition(tw[j], &dec[16 * j]);
-    BTE_HIP(hipMalloc((void**)&t.dec, dec.size() * 4));
-    BTE_HIP(hipMemcpy(t.dec, dec.data(), dec.size() * 4, hipMemcpyHostToDevice));
     std::vector<tw_naf> naf(tw.size());
     for (size_t j = 0; j < tw.size(); ++j) naf[j] = twiddle_naf(tw[j]);
-    BTE_HIP(hipMalloc((void**)&t.naf, naf.size() * sizeof(tw_naf)));
-    BTE_HIP(hipMemcpy(t.naf, naf.data(), naf.size() * sizeof(tw_naf), hipMemcpyHostToDevice));
-    BTE_HIP(hipMalloc((void**)&t.size_inv, 32));
-    BTE_HIP(hipMemcpy(t.size_inv, ninv.l, 32, hipMemcpyHostToDevice));
+    DomainTables nt;
+    hipError_t e = upload_new(&nt.twiddles, (const uint32_t*)tw.data(), tw.size() * 8);
+    if (e == hipSuccess) e = upload_new(&nt.dec, (const uint32_t*)dec.data(), dec.size());
+    if (e == hipSuccess) e = upload_new(&nt.naf, (const tw_naf*)naf.data(), naf.size());
+    if (e == hipSuccess) e = upload_new(&nt.size_inv, (const uint32_t*)ninv.l, 8);
+    if (e != hipSuccess) {
+      for (void* p : {(void*)nt.twiddles, (void*)nt.dec, (void*)nt.naf, (void*)nt.size_inv})
+        if (p) (void)hipFree(p);
+      return set_error(BTE_ENOMEM, "fft domain tables (2^%d): %s", logn, hipGetErrorString(e));
+    }
+    t.dec = nt.dec;
+    t.naf = nt.naf;
+    t.size_inv = nt.size_inv;
+    t.twiddles = nt.twiddles;  // published last: it marks the entry as built
   }
   if (want_plans && !t.plans) {
     std::vector<cyc_plan> pl(half ? half : 1);
@@ -216,10 +242,15 @@ static int domain_tables(int logn, bool inverse, bool want_plans, DomainTables**
       cur = bteh::fr_mul(cur, w);
     }
     cyc_plan np = make_plan(bteh::fr_inv(bteh::fr_from_u64((uint64_t)1 << logn)));
-    BTE_HIP(hipMalloc((void**)&t.plans, pl.size() * sizeof(cyc_plan)));
-    BTE_HIP(hipMemcpy(t.plans, pl.data(), pl.size() * sizeof(cyc_plan), hipMemcpyHostToDevice));
-    BTE_HIP(hipMalloc((void**)&t.size_inv_plan, sizeof(cyc_plan)));
-    BTE_HIP(hipMemcpy(t.size_inv_plan, &np, sizeof(cyc_plan), hipMemcpyHostToDevice));
+    cyc_plan *plans = nullptr, *inv_plan = nullptr;
+    hipError_t e = upload_new(&plans, (const cyc_plan*)pl.data(), pl.size());
+    if (e == hipSuccess) e = upload_new(&inv_plan, (const cyc_plan*)&np, 1);
+    if (e != hipSuccess) {
+      if (plans) (void)hipFree(plans);
+      return set_error(BTE_ENOMEM, "fft domain plans (2^%d): %s", logn, hipGetErrorString(e));
+    }
+    t.size_inv_plan = inv_plan;
+    t.plans = plans;  // published last
   }
   (void)s;
   *out = &t;

@@ -228,10 +228,7 @@ extern "C" int bte_pfe_prepare(const bte_p2_affine* material, size_t total_weigh
     coff.push_back(sel.size());
   }
   const size_t WT = sel.size();
-  std::vector<bteh::Fr> lam(WT), lam_s(WT);
-  PFE_TRY(bte_lagrange_at_zero(W, sel.data(), WT, (uint8_t*)lam.data()));
   const bteh::Fr inv2b = bteh::fr_inv(bteh::fr_from_u64(2 * B));
-  for (size_t k = 0; k < WT; ++k) lam_s[k] = bteh::fr_mul(lam[k], inv2b);
   // d1 MSMs: segment (slot, p) over d1_slot(slot)[range_p] with lambda / 2B
   std::vector<uint32_t> i1(B * WT), si1(B * WT), i2(B * WT), si2(B * WT);
   std::vector<uint64_t> o1(B * T + 1), o2(B + 1);
@@ -259,8 +256,10 @@ extern "C" int bte_pfe_prepare(const bte_p2_affine* material, size_t total_weigh
   uint64_t *d_o1, *d_o2;
   g2j *d1p, *d1t, *d2t, *mask;
   g2a *d1t_aff, *mask_aff;
-  PFE_TRY(sc.upload(&d_lam, (const uint8_t*)lam.data(), 32 * WT));
-  PFE_TRY(sc.upload(&d_lams, (const uint8_t*)lam_s.data(), 32 * WT));
+  PFE_TRY(sc.get(&d_lam, 32 * WT));
+  PFE_TRY(sc.get(&d_lams, 32 * WT));
+  PFE_TRY(lagrange_at_zero_device(W, sel.data(), WT, nullptr, d_lam, s));
+  PFE_TRY(lagrange_at_zero_device(W, sel.data(), WT, (const uint8_t*)inv2b.l, d_lams, s));
   PFE_TRY(sc.upload(&d_i1, i1.data(), i1.size()));
   PFE_TRY(sc.upload(&d_si1, si1.data(), si1.size()));
   PFE_TRY(sc.upload(&d_i2, i2.data(), i2.size()));

@@ -105,6 +105,7 @@ class BtxWorkload:
         n = B * count
         s = self.stream.scalars(n)
         r = self.stream.scalars(n)
+        self.s, self.r = s, r  # the messages' exponents, for checks against gT^s
         ct1 = api.g1_generator_mul(api.to_device(inputs.pack_scalars(r)))
         gts = api.to_device(self.gt * n)
         ct2 = api.cyclotomic_mul_scalar_decomposed(gts, [(a + self.z * b) % R for a, b in zip(s, r)])
