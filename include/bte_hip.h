@@ -45,8 +45,11 @@ const char* bte_version(void);
 /* ---------------------------------------------------------------------------------------- *
  * a1/a2: multi-scalar multiplication, sum_i s_i * P_i over the low 255 bits of each scalar
  * (blst_p{1,2}s_mult_pippenger with nbits = 255).  n == 0 gives the identity.
- * `out` is one Jacobian point in device memory.  These calls synchronise `stream` (the last
- * window combination runs on the host) and are therefore not capturable in a graph.
+ * `out` is one Jacobian point in device memory.  The last window combination (a serial chain of
+ * about 256 doublings) runs as a host callback on `stream` between two stream-ordered copies,
+ * so bte_g1_msm and bte_g2_msm neither wait nor allocate once the stream's workspace has grown;
+ * captured into a graph, the same chain runs in one device thread instead (a few ms).
+ * bte_g2_msm_indexed waits for its stream to report out-of-range indices.
  * ---------------------------------------------------------------------------------------- */
 
 /* weighted_btx/src/blst_utils.rs:43-67 g1_multi_exp_affine_bytes */
@@ -413,6 +416,14 @@ int bte_probe_mac_rate(double* macs_per_second, void* stream);
  * cached tables (FFT domain twiddles and plans) fails as if the device were out of memory (0
  * cancels), so a test can check that a failed build leaves nothing half-made behind. */
 int bte_test_fail_alloc(int nth);
+
+/* Workspace of `stream` (no counterpart in the reference: its calls allocate per call).  Every
+ * entry point takes its device scratch from a grow-only buffer per (device, stream) and stages
+ * host data through pinned buffers per stream, so after the first calls of a shape it neither
+ * allocates nor waits (calls that return host results excepted).  This pre-sizes both: device
+ * bytes of scratch and host bytes per staging buffer.  Graph capture of any entry point is
+ * supported on a stream other than the null stream; captured work owns its own scratch. */
+int bte_stream_reserve(void* stream, size_t device_bytes, size_t host_bytes);
 
 /* ---------------------------------------------------------------------------------------- *
  * Device memory helpers for hosts without a HIP binding (the Rust FFI crate).

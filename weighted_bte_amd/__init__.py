@@ -102,6 +102,7 @@ _SIGNATURES = {
     "bte_memcpy_h2d": [_vp, _vp, _sz, _vp],
     "bte_memcpy_d2h": [_vp, _vp, _sz, _vp],
     "bte_stream_synchronize": [_vp],
+    "bte_stream_reserve": [_vp, _sz, _sz],
     "bte_test_fail_alloc": [_i],
 }
 _RESTYPES = {"bte_last_error": ctypes.c_char_p, "bte_version": ctypes.c_char_p}

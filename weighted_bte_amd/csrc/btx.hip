@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "ctx.hpp"
 #include "curve.hpp"
 #include "cyclo.hpp"
 #include "fr.hpp"
@@ -213,31 +214,11 @@ __global__ void __launch_bounds__(64) k_req_combine(const ReqDev* __restrict__ R
   R[k].out[i] = valid[j] ? fp12_mul(fp12_mul(R[k].ct2[i], fp12_conj(alpha[j])), beta[j]) : fp12_one();
 }
 
-// Stream-ordered scratch that is released when the driver returns.
-struct Scratch {
-  hipStream_t s;
-  std::vector<void*> ptrs;
-  explicit Scratch(hipStream_t st) : s(st) {}
-  ~Scratch() {
-    for (void* p : ptrs) (void)hipFreeAsync(p, s);
-  }
-  template <class T> int get(T** p, size_t count) {
-    hipError_t e = hipMallocAsync((void**)p, (count ? count : 1) * sizeof(T), s);
-    if (e != hipSuccess) return set_error(BTE_ENOMEM, "scratch (%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
-    ptrs.push_back((void*)*p);
-    return BTE_OK;
-  }
-  template <class T> int upload(T** p, const T* host, size_t count) {
-    int rc = get(p, count);
-    if (rc) return rc;
-    BTE_HIP(hipMemcpyAsync(*p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
-    return BTE_OK;
-  }
-  // Host sources of upload() may be temporaries: wait until the copies have consumed them.
-  int uploads_done() {
-    BTE_HIP(hipStreamSynchronize(s));
-    return BTE_OK;
-  }
+// Scratch and uploads of one driver call through the stream's workspace and pinned staging
+// (ctx.hpp): nothing is allocated or waited for once the workspace has grown to this shape.
+struct Scratch : CallScope {
+  using CallScope::CallScope;
+  template <class T> int get(T** p, size_t count) { return dev(p, count); }
 };
 
 #define BTE_TRY(x)        \
@@ -255,23 +236,6 @@ static size_t next_pow2(size_t v) {
 }  // namespace bte
 
 using namespace bte;
-
-// interpolation.rs:31-52 lagrange_at_zero over the first W points of the radix-2 domain of size
-// nextpow2(W); host memory: selected (n u64 indices), out (n canonical 32-byte scalars).  The
-// host interface has no stream: the device path (product tree, more than 64 points) runs on the
-// null stream and is waited for.
-extern "C" int bte_lagrange_at_zero(size_t total_weight, const uint64_t* selected, size_t n, uint8_t* out) {
-  if (!selected || !out) return set_error(BTE_EINVAL, "lagrange_at_zero: null pointer");
-  uint8_t* d = nullptr;
-  if (n) BTE_HIP(hipMalloc((void**)&d, 32 * n));
-  int rc = lagrange_at_zero_device(total_weight, selected, n, nullptr, d, nullptr);
-  if (rc == BTE_OK) {
-    hipError_t e = hipMemcpy(out, d, 32 * n, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) rc = set_error(BTE_EDEVICE, "lagrange_at_zero: %s", hipGetErrorString(e));
-  }
-  if (d) (void)hipFree(d);
-  return rc;
-}
 
 // decryption.rs:427-551 prepare_decryption.  Host inputs: party_offsets (N+1), accepted party
 // indices (T, in the committee order), batch size B <= L.  Device outputs: kernel_lines
@@ -351,7 +315,6 @@ extern "C" int bte_btx_prepare(const bte_p2_affine* material, size_t total_weigh
   BTE_TRY(sc.get(&kernel_aff, m));
   bteh::Fr minv = bteh::fr_inv(bteh::fr_from_u64(m));
   BTE_TRY(sc.upload(&d_minv, (const uint32_t*)minv.l, 8));
-  BTE_TRY(sc.uploads_done());
 
   PhaseTimer pt(s);
   pt.begin();
@@ -400,7 +363,6 @@ extern "C" int bte_btx_precompute_batched(const bte_p1_affine* ct1, const uint8_
   BTE_TRY(sc.get(&conv, m * K));
   BTE_TRY(sc.get(&pref, B * K));
   BTE_TRY(sc.upload(&d_valid, valid, B * K));
-  BTE_TRY(sc.uploads_done());
   PhaseTimer pt(s);
   pt.begin();
   hipLaunchKernelGGL(k_g1_pad, dim3(blocks_for(m * K, 256)), dim3(256), 0, s, (const g1a*)ct1, (const uint8_t*)d_valid, B, m, K,
@@ -461,7 +423,6 @@ extern "C" int bte_btx_open_batched(const bte_p1* shares, size_t n_accepted, con
   BTE_TRY(sc.get(&alpha, K * B));
   BTE_TRY(sc.upload(&d_off, off.data(), K * B + 1));
   BTE_TRY(sc.upload(&d_valid, valid, K * B));
-  BTE_TRY(sc.uploads_done());
   PhaseTimer pt(s);
   pt.begin();
   BTE_TRY(bte_g1_batch_normalize(shares, K * T, (bte_p1_affine*)sa, s));
@@ -562,7 +523,6 @@ extern "C" int bte_btx_decrypt_requests(const bte_btx_request* requests, size_t 
     BTE_TRY(sc.get(&Q, NP));
   BTE_TRY(sc.get(&mm, S));
   BTE_TRY(sc.get(&alpha, S));
-  BTE_TRY(sc.uploads_done());
   PhaseTimer pt(s);
   pt.begin();
   hipLaunchKernelGGL(k_req_pad, dim3(blocks_for(M, 256)), dim3(256), 0, s, (const ReqDev*)d_R,
@@ -631,7 +591,6 @@ extern "C" int bte_btx_open_prepared_batched(const bte_p1* shares, size_t n_acce
   BTE_TRY(sc.get(&ml, K * B));
   BTE_TRY(sc.get(&alpha, K * B));
   BTE_TRY(sc.upload(&d_valid, valid, K * B));
-  BTE_TRY(sc.uploads_done());
   PhaseTimer pt(s);
   pt.begin();
   BTE_TRY(bte_g1_batch_normalize(shares, K * T, (bte_p1_affine*)sa, s));
@@ -688,7 +647,6 @@ extern "C" int bte_btx_partial_decrypt(const bte_p1_affine* ct1, size_t batch_si
   BTE_TRY(sc.upload(&d_pw, (const uint8_t*)pw.data(), 32 * N * B));
   BTE_TRY(sc.upload(&d_pidx, pidx.data(), N * B));
   BTE_TRY(sc.upload(&d_off, off.data(), N + 1));
-  BTE_TRY(sc.uploads_done());
   BTE_TRY(g1_msm_batched_gather(ct1, B, d_pidx, d_pw, nullptr, N * B, d_off, N, 0, sigma_out, s));
   return BTE_OK;
 }

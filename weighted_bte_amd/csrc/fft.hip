@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "ctx.hpp"
 #include "curve.hpp"
 #include "coop12.hpp"
 #include "cyclo.hpp"
@@ -476,8 +477,9 @@ static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool sc
   if (rc) return rc;
   const size_t nbf = n / 2 * count;
   const bool three = nbf < C3_MAX_BUTTERFLIES;
+  CallScope sc(s);
   uint32_t* tab = ext_tab;  // exponentiation tables, C3_TAB_BYTES per lane
-  if (three && !tab) BTE_HIP(hipMallocAsync((void**)&tab, (size_t)c3_blocks(nbf) * 64 * C3_TAB_BYTES, s));
+  if (three && !tab) BTE_TRY_RC(sc.dev((uint8_t**)&tab, (size_t)c3_blocks(nbf) * 64 * C3_TAB_BYTES));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse<fp12>), dim3(blocks_for(n * count, 256)), dim3(256), 0, s, v, n,
                      logn, count);
   const int full_stages = mode == 1 ? logn - 1 : logn;
@@ -497,7 +499,6 @@ static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool sc
       hipLaunchKernelGGL(k_cyc_prefix_last, dim3(blocks_for(out_count * count, 64)), dim3(64), 0, s, v, n,
                          out_count, (const cyc_plan*)t->plans, count);
   }
-  if (tab && !ext_tab) (void)hipFreeAsync(tab, s);
   if (mode == 0 && inverse && scaled)
     hipLaunchKernelGGL(k_cyc_scale, dim3(blocks_for(n * count, 64)), dim3(64), 0, s, v, n * count,
                        (const cyc_plan*)t->size_inv_plan);
@@ -618,18 +619,14 @@ static int group_fft_segmented(jac<F>* v, const int* log_sizes, size_t count, bo
   DomainTables* t;
   int rc = domain_tables(logmax, inverse, false, &t, s);
   if (rc) return rc;
+  CallScope sc(s);
   uint64_t* d_off;
   uint8_t* d_logs;
   uint32_t *d_inv, *d_order;
-  BTE_HIP(hipMallocAsync((void**)&d_off, (count + 1) * 8, s));
-  BTE_HIP(hipMallocAsync((void**)&d_logs, count, s));
-  BTE_HIP(hipMallocAsync((void**)&d_inv, count * 32, s));
-  BTE_HIP(hipMallocAsync((void**)&d_order, std::max<size_t>(order.size(), 1) * 4, s));
-  BTE_HIP(hipMemcpyAsync(d_off, off.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
-  BTE_HIP(hipMemcpyAsync(d_logs, logs.data(), count, hipMemcpyHostToDevice, s));
-  BTE_HIP(hipMemcpyAsync(d_inv, inv.data(), count * 32, hipMemcpyHostToDevice, s));
-  if (!order.empty()) BTE_HIP(hipMemcpyAsync(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
-  BTE_HIP(hipStreamSynchronize(s));  // host vectors are temporaries
+  BTE_TRY_RC(sc.upload(&d_off, off.data(), count + 1));
+  BTE_TRY_RC(sc.upload(&d_logs, logs.data(), count));
+  BTE_TRY_RC(sc.upload(&d_inv, (const uint32_t*)inv.data(), 8 * count));
+  BTE_TRY_RC(sc.upload(&d_order, order.data(), order.size()));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse_seg<jac<F>>), dim3(blocks_for(total, 256)), dim3(256), 0, s, v,
                      (const uint64_t*)d_off, (const uint8_t*)d_logs, (uint32_t)count, total);
   for (int lh = 0; lh < logmax; ++lh) {
@@ -646,10 +643,6 @@ static int group_fft_segmented(jac<F>* v, const int* log_sizes, size_t count, bo
   if (inverse && scaled)
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_group_scale_seg<F>), dim3(blocks_for(total, 64)), dim3(64), 0, s, v,
                        (const uint64_t*)d_off, (uint32_t)count, total, (const uint32_t*)d_inv);
-  (void)hipFreeAsync(d_off, s);
-  (void)hipFreeAsync(d_logs, s);
-  (void)hipFreeAsync(d_inv, s);
-  (void)hipFreeAsync(d_order, s);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
@@ -732,23 +725,20 @@ int cyc_prefix_ifft_segmented(fp12* v, const int* log_sizes, const uint32_t* out
   DomainTables* t;
   int rc = domain_tables(logmax, true, true, &t, s);
   if (rc) return rc;
+  CallScope sc(s);
   uint64_t* d_off;
   uint8_t* d_logs;
   uint32_t* d_order;
-  BTE_HIP(hipMallocAsync((void**)&d_off, (count + 1) * 8, s));
-  BTE_HIP(hipMallocAsync((void**)&d_logs, count, s));
-  BTE_HIP(hipMallocAsync((void**)&d_order, std::max<size_t>(order.size(), 1) * 4, s));
-  BTE_HIP(hipMemcpyAsync(d_off, off.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
-  BTE_HIP(hipMemcpyAsync(d_logs, logs.data(), count, hipMemcpyHostToDevice, s));
-  if (!order.empty()) BTE_HIP(hipMemcpyAsync(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
+  BTE_TRY_RC(sc.upload(&d_off, off.data(), count + 1));
+  BTE_TRY_RC(sc.upload(&d_logs, logs.data(), count));
+  BTE_TRY_RC(sc.upload(&d_order, order.data(), order.size()));
   size_t widest3 = 0;  // widest stage that takes the three-lane form
   for (int lh = 0; lh < logmax; ++lh) {
     const size_t nbf = start[lh + 1] - start[lh];
     if (nbf < C3_MAX_BUTTERFLIES) widest3 = std::max(widest3, nbf);
   }
   uint32_t* tab = nullptr;
-  if (widest3) BTE_HIP(hipMallocAsync((void**)&tab, (size_t)c3_blocks(widest3) * 64 * C3_TAB_BYTES, s));
-  BTE_HIP(hipStreamSynchronize(s));
+  if (widest3) BTE_TRY_RC(sc.dev((uint8_t**)&tab, (size_t)c3_blocks(widest3) * 64 * C3_TAB_BYTES));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bit_reverse_seg<fp12>), dim3(blocks_for(total, 256)), dim3(256), 0, s, v,
                      (const uint64_t*)d_off, (const uint8_t*)d_logs, (uint32_t)count, total);
   for (int lh = 0; lh < logmax; ++lh) {
@@ -762,26 +752,23 @@ int cyc_prefix_ifft_segmented(fp12* v, const int* log_sizes, const uint32_t* out
                          (const uint32_t*)d_order + start[lh], nbf, (const uint64_t*)d_off, (const uint8_t*)d_logs,
                          (uint32_t)count, logmax, lh, (const cyc_plan*)t->plans);
   }
-  if (tab) (void)hipFreeAsync(tab, s);
-  (void)hipFreeAsync(d_off, s);
-  (void)hipFreeAsync(d_logs, s);
-  (void)hipFreeAsync(d_order, s);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
 
-// Plans for caller-given canonical scalars, uploaded to a stream-ordered buffer.
-static int upload_plans(const uint8_t* host_scalars, size_t n, cyc_plan** out, hipStream_t s) {
-  std::vector<cyc_plan> pl(n);
+// Plans for caller-given canonical scalars, built in pinned staging and uploaded into the
+// caller scope's scratch.
+static int upload_plans(CallScope& sc, const uint8_t* host_scalars, size_t n, cyc_plan** out) {
+  cyc_plan* pl;
+  BTE_TRY_RC(sc.host(&pl, n));
   for (size_t i = 0; i < n; ++i) {
     bteh::Fr f;
     memcpy(f.l, host_scalars + 32 * i, 32);
     if (bteh::geq<4>(f.l, bteh::FR_R)) return set_error(BTE_EINVAL, "scalar %zu is not canonical", i);
     pl[i] = make_plan(f);
   }
-  BTE_HIP(hipMallocAsync((void**)out, n * sizeof(cyc_plan), s));
-  BTE_HIP(hipMemcpyAsync(*out, pl.data(), n * sizeof(cyc_plan), hipMemcpyHostToDevice, s));
-  BTE_HIP(hipStreamSynchronize(s));  // pl is a host temporary
+  BTE_TRY_RC(sc.dev(out, n));
+  BTE_HIP(hipMemcpyAsync(*out, pl, n * sizeof(cyc_plan), hipMemcpyHostToDevice, sc.stream()));
   return BTE_OK;
 }
 
@@ -834,12 +821,11 @@ extern "C" int bte_cyclotomic_pow_scalars(bte_fp12* values, const uint8_t* host_
   if (n == 0) return BTE_OK;
   if (!values || !host_scalars) return set_error(BTE_EINVAL, "cyclotomic_pow_scalars: null pointer");
   hipStream_t s = (hipStream_t)stream;
+  CallScope sc(s);
   cyc_plan* plans;
-  int rc = upload_plans(host_scalars, n, &plans, s);
-  if (rc) return rc;
+  BTE_TRY_RC(upload_plans(sc, host_scalars, n, &plans));
   hipLaunchKernelGGL(k_cyc_pow_plans, dim3(blocks_for(n, 64)), dim3(64), 0, s, (fp12*)values, n,
                      (const cyc_plan*)plans);
-  (void)hipFreeAsync(plans, s);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
@@ -865,12 +851,11 @@ __global__ void __launch_bounds__(64) k_cyc_pow_small_tiled(fp12* __restrict__ v
 
 int bte::cyc_pow_scalars_tiled(fp12* v, const uint8_t* host_scalars, size_t B, size_t K, hipStream_t s) {
   if (B == 0 || K == 0) return BTE_OK;
+  CallScope sc(s);
   cyc_plan* plans;
-  int rc = upload_plans(host_scalars, B, &plans, s);
-  if (rc) return rc;
+  BTE_TRY_RC(upload_plans(sc, host_scalars, B, &plans));
   hipLaunchKernelGGL(k_cyc_pow_plans_tiled, dim3(blocks_for(B * K, 64)), dim3(64), 0, s, v, B, K,
                      (const cyc_plan*)plans);
-  (void)hipFreeAsync(plans, s);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
 }

@@ -24,17 +24,11 @@
 #include <vector>
 
 #include "common.hpp"
+#include "ctx.hpp"
 #include "fr.hpp"
 #include "host_field.hpp"
 
 namespace bte {
-
-BTE_DEV bool fr_is_zero(const fr& a) {
-  uint32_t acc = 0;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) acc |= a.l[i];
-  return acc == 0;
-}
 
 BTE_DEV fr fr_neg(const fr& a) {
   fr z;
@@ -293,24 +287,19 @@ __global__ void k_derivative(const fr* __restrict__ P, size_t deg, size_t M, fr*
   }
 }
 
-// out_i = -P(0) / (x_i P'(x_i)) (canonical), scaled by *scale (Montgomery) when given; err set on
-// a zero denominator (repeated points).
+// out_i = -P(0) / (x_i P'(x_i)) (canonical), scaled by *scale (Montgomery).  The selection is
+// validated on the host (distinct points of the domain), so x_i P'(x_i) = x_i prod_{j != i}
+// (x_i - x_j) is never zero.
 __global__ void __launch_bounds__(64) k_lagrange_out(const uint32_t* __restrict__ sel, size_t n, const fr* __restrict__ tw,
                                                      size_t half_m, const fr* __restrict__ der_eval,
                                                      const fr* __restrict__ P, const fr* __restrict__ scale,
-                                                     fr* __restrict__ out, int* __restrict__ err) {
+                                                     fr* __restrict__ out) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t e = sel[i];
   const fr den = fr_mont_mul(domain_point(tw, half_m, e), der_eval[e]);
-  if (fr_is_zero(den)) *err = 1;
-  fr v = fr_mont_mul(fr_neg(P[0]), fr_mont_inv(den));
-  if (scale) v = fr_mont_mul(v, *scale);
+  const fr v = fr_mont_mul(fr_mont_mul(fr_neg(P[0]), fr_mont_inv(den)), *scale);
   out[i] = fr_from_mont(v);
-}
-
-__global__ void k_fr_zero_flag(int* f) {
-  if (threadIdx.x == 0) *f = 0;
 }
 
 __global__ void k_fr_fill_canonical(fr* out, size_t n, fr value) {
@@ -351,22 +340,6 @@ static void lagrange_direct_host(int logm, const uint64_t* sel, size_t n, std::v
   }
 }
 
-// Stream-ordered temporaries released when the call returns.
-struct FrScratch {
-  hipStream_t s;
-  std::vector<void*> ptrs;
-  explicit FrScratch(hipStream_t st) : s(st) {}
-  ~FrScratch() {
-    for (void* p : ptrs) (void)hipFreeAsync(p, s);
-  }
-  template <class T> int get(T** p, size_t count) {
-    hipError_t e = hipMallocAsync((void**)p, (count ? count : 1) * sizeof(T), s);
-    if (e != hipSuccess) return set_error(BTE_ENOMEM, "lagrange scratch: %s", hipGetErrorString(e));
-    ptrs.push_back((void*)*p);
-    return BTE_OK;
-  }
-};
-
 int lagrange_at_zero_device(size_t W, const uint64_t* sel, size_t n, const uint8_t* scale_canonical,
                             uint8_t* d_out, hipStream_t s) {
   BTE_TRY_RC(validate_selection(W, sel, n));
@@ -387,8 +360,11 @@ int lagrange_at_zero_device(size_t W, const uint64_t* sel, size_t n, const uint8
     std::vector<bteh::Fr> c;
     lagrange_direct_host(logm, sel, n, c);
     for (auto& v : c) v = bteh::fr_mul(v, scale);
-    BTE_HIP(hipMemcpyAsync(d_out, c.data(), 32 * n, hipMemcpyHostToDevice, s));
-    BTE_HIP(hipStreamSynchronize(s));  // c is a temporary
+    CallScope sc(s);
+    void* stage;
+    BTE_TRY_RC(sc.host_bytes(&stage, 32 * n));
+    memcpy(stage, c.data(), 32 * n);
+    BTE_HIP(hipMemcpyAsync(d_out, stage, 32 * n, hipMemcpyHostToDevice, s));
     return BTE_OK;
   }
   // Tree levels: offsets of every node of every level (host), uploaded once.
@@ -431,25 +407,19 @@ int lagrange_at_zero_device(size_t W, const uint64_t* sel, size_t n, const uint8
   }
   const FrDomain* dom;
   BTE_TRY_RC(fr_domain(logm, false, &dom));
-  FrScratch sc(s);
+  CallScope sc(s);
   uint64_t* d_offs;
   uint32_t* d_sel;
   fr *lv[2], *buf, *prod, *der, *d_scale;
-  int* d_err;
-  BTE_TRY_RC(sc.get(&d_offs, flat.size()));
-  BTE_TRY_RC(sc.get(&d_sel, n));
-  BTE_TRY_RC(sc.get(&lv[0], maxcoef));
-  BTE_TRY_RC(sc.get(&lv[1], maxcoef));
-  BTE_TRY_RC(sc.get(&buf, maxT));
-  BTE_TRY_RC(sc.get(&prod, maxT / 2 + 1));
-  BTE_TRY_RC(sc.get(&der, M));
-  BTE_TRY_RC(sc.get(&d_scale, 1));
-  BTE_TRY_RC(sc.get(&d_err, 1));
   const bteh::Fr scale_m = host_to_mont(scale);
-  BTE_HIP(hipMemcpyAsync(d_offs, flat.data(), flat.size() * 8, hipMemcpyHostToDevice, s));
-  BTE_HIP(hipMemcpyAsync(d_sel, sel32.data(), n * 4, hipMemcpyHostToDevice, s));
-  BTE_HIP(hipMemcpyAsync(d_scale, &scale_m, 32, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_fr_zero_flag, dim3(1), dim3(64), 0, s, d_err);
+  BTE_TRY_RC(sc.upload(&d_offs, flat.data(), flat.size()));
+  BTE_TRY_RC(sc.upload(&d_sel, sel32.data(), n));
+  BTE_TRY_RC(sc.upload(&d_scale, (const fr*)&scale_m, 1));
+  BTE_TRY_RC(sc.dev(&lv[0], maxcoef));
+  BTE_TRY_RC(sc.dev(&lv[1], maxcoef));
+  BTE_TRY_RC(sc.dev(&buf, maxT));
+  BTE_TRY_RC(sc.dev(&prod, maxT / 2 + 1));
+  BTE_TRY_RC(sc.dev(&der, M));
   const size_t half_m = M / 2;
   hipLaunchKernelGGL(k_tree_leaves, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const uint32_t*)d_sel, n,
                      (const fr*)dom->tw, half_m, lv[0]);
@@ -486,12 +456,8 @@ int lagrange_at_zero_device(size_t W, const uint64_t* sel, size_t n, const uint8
   hipLaunchKernelGGL(k_derivative, dim3(blocks_for(M, 256)), dim3(256), 0, s, P, n, M, der);
   BTE_TRY_RC(fr_ntt(der, M, 1, false, false, s));
   hipLaunchKernelGGL(k_lagrange_out, dim3(blocks_for(n, 64)), dim3(64), 0, s, (const uint32_t*)d_sel, n,
-                     (const fr*)dom->tw, half_m, (const fr*)der, P, (const fr*)d_scale, (fr*)d_out, d_err);
+                     (const fr*)dom->tw, half_m, (const fr*)der, P, (const fr*)d_scale, (fr*)d_out);
   BTE_CHECK_LAUNCH();
-  int herr = 0;
-  BTE_HIP(hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, s));
-  BTE_HIP(hipStreamSynchronize(s));  // the host arrays above are temporaries; herr is read next
-  if (herr) return set_error(BTE_EINVAL, "InvalidInterpolationSet");
   return BTE_OK;
 }
 
@@ -517,6 +483,36 @@ extern "C" int bte_fft_fr(uint8_t* values, size_t n, int inverse, int scaled, vo
 
 extern "C" int bte_fft_fr_batched(uint8_t* values, size_t n, size_t count, int inverse, int scaled, void* stream) {
   return fft_fr_entry(values, n, count, inverse, scaled, (hipStream_t)stream);
+}
+
+// interpolation.rs:31-52 lagrange_at_zero with host output.  Complete power-of-two domains and
+// sets of up to 64 points are computed on the host; larger sets take the device product tree on
+// the null stream (the host interface has no stream), which this call waits for.
+extern "C" int bte_lagrange_at_zero(size_t total_weight, const uint64_t* selected, size_t n, uint8_t* out) {
+  if (!selected || !out) return set_error(BTE_EINVAL, "lagrange_at_zero: null pointer");
+  BTE_TRY_RC(validate_selection(total_weight, selected, n));
+  int logm = 0;
+  while (((size_t)1 << logm) < total_weight) ++logm;
+  if (n == total_weight && ((size_t)1 << logm) == total_weight) {
+    const bteh::Fr v = bteh::fr_inv(bteh::fr_from_u64(total_weight));
+    for (size_t i = 0; i < n; ++i) memcpy(out + 32 * i, v.l, 32);
+    return BTE_OK;
+  }
+  if (n <= LAGRANGE_DIRECT_MAX) {
+    std::vector<bteh::Fr> c;
+    lagrange_direct_host(logm, selected, n, c);
+    memcpy(out, c.data(), 32 * n);
+    return BTE_OK;
+  }
+  uint8_t* d = nullptr;
+  BTE_HIP(hipMalloc((void**)&d, 32 * n));
+  int rc = lagrange_at_zero_device(total_weight, selected, n, nullptr, d, nullptr);
+  if (rc == BTE_OK) {
+    const hipError_t e = hipMemcpy(out, d, 32 * n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = set_error(BTE_EDEVICE, "lagrange_at_zero: %s", hipGetErrorString(e));
+  }
+  (void)hipFree(d);
+  return rc;
 }
 
 extern "C" int bte_lagrange_at_zero_device(size_t total_weight, const uint64_t* selected, size_t n, uint8_t* out,

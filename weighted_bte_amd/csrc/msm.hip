@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "ctx.hpp"
 #include "curve.hpp"
 #include "host_field.hpp"
 
@@ -608,57 +609,6 @@ __global__ void __launch_bounds__(64) k_msm_combine(const jac<F>* __restrict__ w
 // ---------------------------------------------------------------------------------------------
 // Host driver
 // ---------------------------------------------------------------------------------------------
-// Grow-only device workspace per (device, stream).  An MSM call holds its entry's lock from
-// the first launch to its final stream synchronisation, so calls on one stream serialise and
-// calls on different streams use different memory.
-struct Workspace {
-  std::mutex mu;
-  void* base = nullptr;
-  size_t cap = 0;
-};
-static std::mutex g_ws_mu;
-static std::map<std::pair<int, hipStream_t>, Workspace*> g_ws;
-
-static Workspace* workspace_for(hipStream_t s) {
-  int dev = 0;
-  (void)hipGetDevice(&dev);
-  std::lock_guard<std::mutex> lk(g_ws_mu);
-  Workspace*& w = g_ws[std::make_pair(dev, s)];
-  if (!w) w = new Workspace();
-  return w;
-}
-
-// Carves typed, 256-byte aligned sub-buffers out of one allocation.
-struct Carver {
-  size_t off = 0;
-  std::vector<std::pair<void**, size_t>> items;
-  template <class T> void add(T** p, size_t count) {
-    items.push_back({(void**)p, (count ? count : 1) * sizeof(T)});
-  }
-  hipStream_t stream;
-  explicit Carver(hipStream_t s) : stream(s) {}
-  int commit(Workspace* ws) {
-    size_t total = 0;
-    for (auto& it : items) total += (it.second + 255) & ~(size_t)255;
-    if (total > ws->cap) {
-      // earlier asynchronous calls on this stream may still use the old buffer
-      BTE_HIP(hipStreamSynchronize(stream));
-      if (ws->base) BTE_HIP(hipFree(ws->base));
-      ws->base = nullptr;
-      ws->cap = 0;
-      hipError_t e = hipMalloc(&ws->base, total);
-      if (e != hipSuccess) return set_error(BTE_ENOMEM, "msm workspace (%zu bytes): %s", total, hipGetErrorString(e));
-      ws->cap = total;
-    }
-    char* b = (char*)ws->base;
-    for (auto& it : items) {
-      *it.first = b;
-      b += (it.second + 255) & ~(size_t)255;
-    }
-    return BTE_OK;
-  }
-};
-
 // Accumulation threads: ACC_ROUNDS full-occupancy rounds of one-wavefront workgroups on this
 // device (queried once per device), and no more than chunks of the minimum size need.
 template <class F>
@@ -718,14 +668,78 @@ static void lds_sort_pass(bool scatter, unsigned nwg, const uint32_t* scalars, s
     lds_hist_launch<32768, 1024>(scatter, nwg, scalars, n, c, M, S, P, hist, offsets, sorted, s);
 }
 
+// Host tail of a single MSM: the window sums come back into pinned memory, a host callback on
+// the stream combines them (Horner over the exponent: A_w sits at 2^(c w), D_{w,b} at
+// 2^(c w + logL + b)) and the result goes back to the device, all in stream order, so the call
+// neither waits nor allocates and can be captured into a graph.
+template <class HF> struct HornerJob {
+  int c, nw, logL, logS, err;
+  bteh::Jac<HF> result;
+  bteh::Jac<HF>* hw;  // nw x (2 + logS) sums
+};
+
+template <class HF> static void horner_host(void* data) {
+  HornerJob<HF>* j = (HornerJob<HF>*)data;
+  const int nsum = 2 + j->logS, top = j->c * j->nw;
+  std::vector<std::vector<int>> at(top + 1);
+  for (int w = 0; w < j->nw; ++w) {
+    at[j->c * w].push_back(w * nsum);
+    at[j->c * w].push_back(w * nsum + 1);
+    for (int b = 0; b < j->logS; ++b) at[j->c * w + j->logL + b].push_back(w * nsum + 2 + b);
+  }
+  bteh::Jac<HF> acc = bteh::jac_identity<HF>();
+  for (int e = top; e >= 0; --e) {
+    acc = bteh::jac_dbl(acc);
+    for (int k : at[e]) acc = bteh::jac_add(acc, j->hw[k]);
+  }
+  j->result = acc;
+}
+
+template <class F> __global__ void k_msm_identity(jac<F>* out) {
+  if (threadIdx.x == 0) *out = jac_identity<F>();
+}
+
+// The same Horner pass on the device, one thread, in the host's order of operations (so the same
+// Jacobian coordinates): used when the stream is being captured into a graph, where the host
+// callback form is not relied on.  About 256 doublings and 240 additions in one lane: a few
+// milliseconds, against tenths of a millisecond on a host core.
+template <class F>
+__global__ void __launch_bounds__(64) k_msm_horner(const jac<F>* __restrict__ wsum, int c, int nw, int logL, int logS,
+                                                   jac<F>* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  const int nsum = 2 + logS, top = c * nw;
+  jac<F> acc = jac_identity<F>();
+  for (int e = top; e >= 0; --e) {
+    acc = jac_dbl(acc);
+    const int w = e / c, r = e - w * c, b = r - logL;
+    if (w >= nw) continue;
+    if (r == 0) {
+      acc = jac_add(acc, wsum[w * nsum]);
+      acc = jac_add(acc, wsum[w * nsum + 1]);
+    }
+    if (b >= 0 && b < logS) acc = jac_add(acc, wsum[w * nsum + 2 + b]);
+  }
+  *out = acc;
+}
+
+// The host callback's result, read from pinned host memory when the kernel runs.  (A host-to-
+// device copy node of a captured graph may take its bytes when it is recorded, before any
+// callback has run; a kernel argument holding the address reads them at replay time.)
+template <class F> __global__ void k_msm_result_from_host(const jac<F>* __restrict__ host, jac<F>* __restrict__ out) {
+  constexpr int W = sizeof(jac<F>) / 4;
+  const volatile uint32_t* src = (const volatile uint32_t*)host;
+  uint32_t* dst = (uint32_t*)out;
+  for (int i = threadIdx.x; i < W; i += blockDim.x) dst[i] = src[i];
+}
+
 template <class F>
 static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint32_t* index,
                    const uint8_t* scalars, size_t n, jac<F>* out, hipStream_t s) {
   using HF = typename MsmTraits<F>::HF;
   if (n == 0) {
-    bteh::Jac<HF> id = bteh::jac_identity<HF>();
-    BTE_HIP(hipMemcpyAsync(out, &id, sizeof(id), hipMemcpyHostToDevice, s));
-    BTE_HIP(hipStreamSynchronize(s));
+    if (!out) return set_error(BTE_EINVAL, "msm: null pointer");
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_identity<F>), dim3(1), dim3(64), 0, s, out);
+    BTE_CHECK_LAUNCH();
     return BTE_OK;
   }
   if (!points || !scalars || !out) return set_error(BTE_EINVAL, "msm: null pointer");
@@ -744,38 +758,43 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
   const bool lds_sort = sh.M <= 32768;
   const uint32_t S_sort = lds_sort ? sort_slices(n, sh.nw, sh.M) : 0;
 
-  Workspace* ws = workspace_for(s);
-  std::lock_guard<std::mutex> ws_lock(ws->mu);
-  Carver cv(s);
+  CallScope sc(s);
   int32_t* digits = nullptr;
   uint32_t *counts, *offsets, *cursor = nullptr, *sorted, *tile_sums, *hist = nullptr;
   jac<F> *buckets, *heads, *tails, *seg_acc, *seg_run, *wsum;
   int* err;
   uint32_t *big, *nbig, *mid, *nmid;
   const size_t max_big = tstar / FIX_SERIAL + 1;
-  int rc;
   if (lds_sort) {
-    cv.add(&hist, (size_t)sh.nw * S_sort * sh.M);
+    BTE_TRY_RC(sc.dev(&hist, (size_t)sh.nw * S_sort * sh.M));
   } else {
-    cv.add(&digits, E_max);
-    cv.add(&cursor, nb);
+    BTE_TRY_RC(sc.dev(&digits, E_max));
+    BTE_TRY_RC(sc.dev(&cursor, nb));
   }
-  cv.add(&counts, nb);
-  cv.add(&offsets, (size_t)nb + 1);
-  cv.add(&sorted, E_max);
-  cv.add(&tile_sums, ntiles);
-  cv.add(&buckets, nb);
-  cv.add(&heads, tstar);
-  cv.add(&tails, tstar);
-  cv.add(&seg_acc, sh.nw * S);
-  cv.add(&seg_run, sh.nw * S);
-  cv.add(&wsum, (size_t)sh.nw * nsum);
-  cv.add(&err, 1);
-  cv.add(&big, max_big);
-  cv.add(&nbig, 1);
-  cv.add(&mid, nb);
-  cv.add(&nmid, 1);
-  if ((rc = cv.commit(ws))) return rc;
+  BTE_TRY_RC(sc.dev(&counts, nb));
+  BTE_TRY_RC(sc.dev(&offsets, (size_t)nb + 1));
+  BTE_TRY_RC(sc.dev(&sorted, E_max));
+  BTE_TRY_RC(sc.dev(&tile_sums, ntiles));
+  BTE_TRY_RC(sc.dev(&buckets, nb));
+  BTE_TRY_RC(sc.dev(&heads, tstar));
+  BTE_TRY_RC(sc.dev(&tails, tstar));
+  BTE_TRY_RC(sc.dev(&seg_acc, sh.nw * S));
+  BTE_TRY_RC(sc.dev(&seg_run, sh.nw * S));
+  BTE_TRY_RC(sc.dev(&wsum, (size_t)sh.nw * nsum));
+  BTE_TRY_RC(sc.dev(&err, 1));
+  BTE_TRY_RC(sc.dev(&big, max_big));
+  BTE_TRY_RC(sc.dev(&nbig, 1));
+  BTE_TRY_RC(sc.dev(&mid, nb));
+  BTE_TRY_RC(sc.dev(&nmid, 1));
+  HornerJob<HF>* job = nullptr;
+  if (!sc.capturing()) {
+    BTE_TRY_RC(sc.host(&job, 1));
+    BTE_TRY_RC(sc.host(&job->hw, (size_t)sh.nw * nsum));
+    job->c = sh.c;
+    job->nw = sh.nw;
+    job->logL = sh.logL;
+    job->logS = sh.logS;
+  }
 
   PhaseTimer pt(s);
   pt.begin();
@@ -829,30 +848,28 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
                      (const jac<F>*)seg_acc, (const jac<F>*)seg_run, sh.logS, wsum);
   pt.mark("window_sums");
   BTE_CHECK_LAUNCH();
-
-  std::vector<bteh::Jac<HF>> hw((size_t)sh.nw * nsum);
-  int herr = 0;
-  BTE_HIP(hipMemcpyAsync(hw.data(), wsum, hw.size() * sizeof(hw[0]), hipMemcpyDeviceToHost, s));
-  BTE_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
-  BTE_HIP(hipStreamSynchronize(s));
-  pt.publish();
-  if (herr) return set_error(BTE_EINVAL, "msm: point index out of range");
-
-  // Horner over the exponent: A_w sits at 2^(c w), D_{w,b} at 2^(c w + logL + b).
-  const int top = sh.c * sh.nw;
-  std::vector<std::vector<int>> at(top + 1);
-  for (int w = 0; w < sh.nw; ++w) {
-    at[sh.c * w].push_back(w * nsum);
-    at[sh.c * w].push_back(w * nsum + 1);
-    for (int b = 0; b < sh.logS; ++b) at[sh.c * w + sh.logL + b].push_back(w * nsum + 2 + b);
+  if (sc.capturing()) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_horner<F>), dim3(1), dim3(64), 0, s, (const jac<F>*)wsum, sh.c, sh.nw,
+                       sh.logL, sh.logS, out);
+    BTE_CHECK_LAUNCH();
+    return BTE_OK;
   }
-  bteh::Jac<HF> acc = bteh::jac_identity<HF>();
-  for (int e = top; e >= 0; --e) {
-    acc = bteh::jac_dbl(acc);
-    for (int k : at[e]) acc = bteh::jac_add(acc, hw[k]);
+  BTE_HIP(hipMemcpyAsync(job->hw, wsum, (size_t)sh.nw * nsum * sizeof(bteh::Jac<HF>), hipMemcpyDeviceToHost, s));
+  BTE_HIP(hipMemcpyAsync(&job->err, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  BTE_HIP(hipLaunchHostFunc(s, horner_host<HF>, job));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_result_from_host<F>), dim3(1), dim3(64), 0, s,
+                     (const jac<F>*)&job->result, out);
+  BTE_CHECK_LAUNCH();
+  if (pt.on) {
+    BTE_HIP(hipStreamSynchronize(s));
+    pt.publish();
   }
-  BTE_HIP(hipMemcpyAsync(out, &acc, sizeof(acc), hipMemcpyHostToDevice, s));
-  BTE_HIP(hipStreamSynchronize(s));
+  // Gathered points: an index out of range is reported by the call, which therefore waits for
+  // its stream (outside graph capture).
+  if (index && !sc.capturing()) {
+    BTE_HIP(hipStreamSynchronize(s));
+    if (job->err) return set_error(BTE_EINVAL, "msm: point index out of range");
+  }
   return BTE_OK;
 }
 
@@ -896,34 +913,30 @@ static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t
   const int ntiles = (int)((nb + 1023) / 1024);
   const size_t max_big = tstar / FIX_SERIAL + 1;
 
-  Workspace* ws = workspace_for(s);
-  std::lock_guard<std::mutex> ws_lock(ws->mu);
-  Carver cv(s);
+  CallScope cv(s);
   uint32_t *keys, *counts, *offsets, *cursor, *sorted, *tile_sums, *big, *nbig, *mid, *nmid;
   jac<F> *buckets, *heads, *tails, *win, *seg_acc, *seg_run;
   int* err;
   // bucket reduction in two levels: runs of L = 2^logL buckets per thread, then per window
   const int logL = std::min(3, c - 1), logS = (c - 1) - logL;
   const size_t nsw = (size_t)nseg * nw, nss = nsw << logS;
-  cv.add(&keys, E_max);
-  cv.add(&counts, nb);
-  cv.add(&offsets, (size_t)nb + 1);
-  cv.add(&cursor, nb);
-  cv.add(&sorted, E_max);
-  cv.add(&tile_sums, ntiles);
-  cv.add(&buckets, nb);
-  cv.add(&heads, tstar);
-  cv.add(&tails, tstar);
-  cv.add(&win, nsw);
-  cv.add(&seg_acc, nss);
-  cv.add(&seg_run, nss);
-  cv.add(&err, 1);
-  cv.add(&big, max_big);
-  cv.add(&nbig, 1);
-  cv.add(&mid, nb);
-  cv.add(&nmid, 1);
-  int rc = cv.commit(ws);
-  if (rc) return rc;
+  BTE_TRY_RC(cv.dev(&keys, E_max));
+  BTE_TRY_RC(cv.dev(&counts, nb));
+  BTE_TRY_RC(cv.dev(&offsets, (size_t)nb + 1));
+  BTE_TRY_RC(cv.dev(&cursor, nb));
+  BTE_TRY_RC(cv.dev(&sorted, E_max));
+  BTE_TRY_RC(cv.dev(&tile_sums, ntiles));
+  BTE_TRY_RC(cv.dev(&buckets, nb));
+  BTE_TRY_RC(cv.dev(&heads, tstar));
+  BTE_TRY_RC(cv.dev(&tails, tstar));
+  BTE_TRY_RC(cv.dev(&win, nsw));
+  BTE_TRY_RC(cv.dev(&seg_acc, nss));
+  BTE_TRY_RC(cv.dev(&seg_run, nss));
+  BTE_TRY_RC(cv.dev(&err, 1));
+  BTE_TRY_RC(cv.dev(&big, max_big));
+  BTE_TRY_RC(cv.dev(&nbig, 1));
+  BTE_TRY_RC(cv.dev(&mid, nb));
+  BTE_TRY_RC(cv.dev(&nmid, 1));
 
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, counts, (size_t)nb, 0u);
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nb, 256)), dim3(256), 0, s, cursor, (size_t)nb, 0u);

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "ctx.hpp"
 #include "curve.hpp"
 #include "cyclo.hpp"
 
@@ -307,14 +308,14 @@ extern "C" int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine
   if (!seg_offsets || !out || (npairs && (!p || !q)))
     return set_error(BTE_EINVAL, "multi_miller_loop: null pointer");
   hipStream_t s = (hipStream_t)stream;
+  CallScope sc(s);
   fp12* f = nullptr;
-  BTE_HIP(hipMallocAsync((void**)&f, (npairs ? npairs : 1) * sizeof(fp12), s));
+  BTE_TRY_RC(sc.dev(&f, npairs));
   if (npairs)
     hipLaunchKernelGGL(k_miller_pairs, dim3(blocks_for(npairs, 64)), dim3(64), 0, s, (const g1a*)p,
                        (const g2a*)q, npairs, f);
   hipLaunchKernelGGL(k_segment_product, dim3((unsigned)nseg), dim3(64), 0, s, (const fp12*)f, seg_offsets, nseg,
                      (fp12*)out);
-  (void)hipFreeAsync(f, s);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
@@ -359,23 +360,12 @@ int bte::multi_miller_prepared_ranges(const void* const* pair_tables, const void
   const size_t nchunks = ch.lo.size();
   uint64_t *d_lo, *d_hi, *d_seg;
   fp12* part;
-  const size_t nc = std::max<size_t>(nchunks, 1);
-  BTE_HIP(hipMallocAsync((void**)&d_lo, nc * 8, s));
-  BTE_HIP(hipMallocAsync((void**)&d_hi, nc * 8, s));
-  BTE_HIP(hipMallocAsync((void**)&d_seg, ch.seg.size() * 8, s));
-  BTE_HIP(hipMallocAsync((void**)&part, nc * sizeof(fp12), s));
-  if (nchunks) {
-    BTE_HIP(hipMemcpyAsync(d_lo, ch.lo.data(), nchunks * 8, hipMemcpyHostToDevice, s));
-    BTE_HIP(hipMemcpyAsync(d_hi, ch.hi.data(), nchunks * 8, hipMemcpyHostToDevice, s));
-  }
-  BTE_HIP(hipMemcpyAsync(d_seg, ch.seg.data(), ch.seg.size() * 8, hipMemcpyHostToDevice, s));
-  BTE_HIP(hipStreamSynchronize(s));  // host vectors are temporaries
-  int rc = mm_enqueue(pair_tables, p, d_lo, d_hi, nchunks, d_seg, nseg, part, out, s);
-  (void)hipFreeAsync(d_lo, s);
-  (void)hipFreeAsync(d_hi, s);
-  (void)hipFreeAsync(d_seg, s);
-  (void)hipFreeAsync(part, s);
-  return rc;
+  CallScope sc(s);
+  BTE_TRY_RC(sc.upload(&d_lo, ch.lo.data(), nchunks));
+  BTE_TRY_RC(sc.upload(&d_hi, ch.hi.data(), nchunks));
+  BTE_TRY_RC(sc.upload(&d_seg, ch.seg.data(), ch.seg.size()));
+  BTE_TRY_RC(sc.dev(&part, nchunks));
+  return mm_enqueue(pair_tables, p, d_lo, d_hi, nchunks, d_seg, nseg, part, out, s);
 }
 
 int bte::multi_miller_prepared(const void* const* pair_tables, const void* p, size_t npairs, const uint64_t* seg_offsets,

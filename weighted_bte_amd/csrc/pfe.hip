@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "ctx.hpp"
 #include "curve.hpp"
 #include "cyclo.hpp"
 #include "host_field.hpp"
@@ -176,28 +177,11 @@ __global__ void __launch_bounds__(64) k_pfe_unmask(const fp12* __restrict__ ct2,
   if (i < n) out[i] = fp12_mul(ct2[i], fp12_conj(mask[i]));
 }
 
-struct PScratch {
-  hipStream_t s;
-  std::vector<void*> ptrs;
-  explicit PScratch(hipStream_t st) : s(st) {}
-  ~PScratch() {
-    for (void* p : ptrs) (void)hipFreeAsync(p, s);
-  }
-  template <class T> int get(T** p, size_t count) {
-    hipError_t e = hipMallocAsync((void**)p, (count ? count : 1) * sizeof(T), s);
-    if (e != hipSuccess) return set_error(BTE_ENOMEM, "scratch: %s", hipGetErrorString(e));
-    ptrs.push_back((void*)*p);
-    return BTE_OK;
-  }
-  template <class T> int upload(T** p, const T* host, size_t count) {
-    PFE_TRY(get(p, count));
-    BTE_HIP(hipMemcpyAsync(*p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
-    return BTE_OK;
-  }
-  int done() {
-    BTE_HIP(hipStreamSynchronize(s));
-    return BTE_OK;
-  }
+// Scratch and uploads of one driver call through the stream's workspace and pinned staging.
+struct PScratch : CallScope {
+  using CallScope::CallScope;
+  template <class T> int get(T** p, size_t count) { return dev(p, count); }
+  int done() { return BTE_OK; }
 };
 
 }  // namespace bte

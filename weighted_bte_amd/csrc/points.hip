@@ -3,6 +3,7 @@
 #include <mutex>
 
 #include "common.hpp"
+#include "ctx.hpp"
 #include "curve.hpp"
 
 namespace bte {
@@ -145,13 +146,12 @@ template <class F> static int generator_mul(const uint8_t* scalars, size_t n, af
   const aff<F>* table;
   int rc = generator_table<F>(&table, s);
   if (rc) return rc;
+  CallScope sc(s);
   jac<F>* tmp;
-  hipError_t e = hipMallocAsync((void**)&tmp, n * sizeof(jac<F>), s);
-  if (e != hipSuccess) return set_error(BTE_ENOMEM, "generator_mul: %s", hipGetErrorString(e));
+  BTE_TRY_RC(sc.dev(&tmp, n));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fixed_base_mul<F>), dim3(blocks_for(n, 64)), dim3(64), 0, s, table,
                      (const uint32_t*)scalars, n, tmp);
   rc = batch_normalize<F>(tmp, n, out, s);
-  (void)hipFreeAsync(tmp, s);
   if (rc) return rc;
   BTE_CHECK_LAUNCH();
   return BTE_OK;

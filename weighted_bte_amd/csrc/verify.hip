@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "ctx.hpp"
 #include "curve.hpp"
 #include "cyclo.hpp"
 #include "host_field.hpp"
@@ -93,25 +94,11 @@ __global__ void __launch_bounds__(64) k_schnorr_check(const g1a* __restrict__ sG
   ok[i] = jac_is_identity(d) ? 1 : 0;
 }
 
-struct VScratch {
-  hipStream_t s;
-  std::vector<void*> ptrs;
-  explicit VScratch(hipStream_t st) : s(st) {}
-  ~VScratch() {
-    for (void* p : ptrs) (void)hipFreeAsync(p, s);
-  }
-  template <class T> int get(T** p, size_t count) {
-    hipError_t e = hipMallocAsync((void**)p, (count ? count : 1) * sizeof(T), s);
-    if (e != hipSuccess) return set_error(BTE_ENOMEM, "verify scratch: %s", hipGetErrorString(e));
-    ptrs.push_back((void*)*p);
-    return BTE_OK;
-  }
-  template <class T> int upload(T** p, const T* host, size_t count) {
-    int rc = get(p, count);
-    if (rc) return rc;
-    if (count) BTE_HIP(hipMemcpyAsync(*p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
-    return BTE_OK;
-  }
+// Scratch and uploads through the stream's workspace and pinned staging (ctx.hpp).  These
+// checks return their verdicts in host memory, so each call ends by waiting for its stream.
+struct VScratch : CallScope {
+  using CallScope::CallScope;
+  template <class T> int get(T** p, size_t count) { return dev(p, count); }
 };
 
 #define V_TRY(x)          \
