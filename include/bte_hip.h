@@ -182,6 +182,14 @@ int bte_miller_loop_pairs(const bte_p1_affine* p, const bte_p2_affine* q, size_t
  * offsets, seg_offsets[nseg] = npairs); pairs with an identity are skipped, empty = one. */
 int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine* q, size_t npairs,
                           const uint64_t* seg_offsets, size_t nseg, bte_fp12* out, void* stream);
+/* No counterpart in the reference: bte_multi_miller_loop for callers that cannot allocate (work
+ * under graph capture, tight serving loops, harnesses that forbid allocation).  The same kernels
+ * and results; the per-pair Miller values live in a file-scope device array of 32 MiB (at most
+ * 58,254 pairs; more is BTE_EINVAL) that each GPU the library runs on holds until the process
+ * ends.  Nothing is allocated, freed or waited for.  Calls that overlap on one GPU share that
+ * array and race: issue them in one stream order. */
+int bte_multi_miller_loop_noalloc(const bte_p1_affine* p, const bte_p2_affine* q, size_t npairs,
+                                  const uint64_t* seg_offsets, size_t nseg, bte_fp12* out, void* stream);
 /* unprepared_multi_miller_loop with the G2 side prepared: pair g uses the line table
  * pair_tables[g] (DEVICE array of npairs pointers to 68-line tables) and p[g]; seg_offsets is a
  * HOST array (nseg+1, from 0 to npairs).  The pairs of a segment share the accumulator and its

@@ -49,6 +49,7 @@ _SIGNATURES = {
     "bte_miller_loop_prepared": [_vp, _vp, _sz, _vp, _vp],
     "bte_miller_loop_pairs": [_vp, _vp, _sz, _vp, _vp],
     "bte_multi_miller_loop": [_vp, _vp, _sz, _vp, _sz, _vp, _vp],
+    "bte_multi_miller_loop_noalloc": [_vp, _vp, _sz, _vp, _sz, _vp, _vp],
     "bte_final_exp_easy": [_vp, _sz, _vp, _vp],
     "bte_final_exp_hard": [_vp, _sz, _vp, _vp],
     "bte_final_exp": [_vp, _sz, _vp, _vp],

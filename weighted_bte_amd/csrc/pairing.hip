@@ -22,6 +22,7 @@
 // shared squaring distributes over the product), so pairs run one per thread and each segment
 // is multiplied together by one workgroup.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -249,6 +250,33 @@ __global__ void __launch_bounds__(64) k_final_exp_hard(const fp12* in, size_t n,
   out[i] = hard_part(in[i]);
 }
 
+// Scratch of bte_multi_miller_loop_noalloc: one Fp12 per pair in a file-scope device array.  Each
+// GPU the library runs on holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress gives on that device (the array's name is a host-side
+// stand-in, not a device address).
+static constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static constexpr size_t NOALLOC_MAX_PAIRS = NOALLOC_SCRATCH_BYTES / sizeof(fp12);
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];
+
+// Device address of `scratch` on the current device, looked up once per device.
+static int scratch_address(void** out) {
+  int dev = 0;
+  BTE_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_error(BTE_EDEVICE, "multi_miller_loop_noalloc: device id %d", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_error(BTE_EDEVICE, "multi_miller_loop_noalloc: scratch address: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return BTE_OK;
+}
+
 }  // namespace bte
 
 using namespace bte;
@@ -314,6 +342,31 @@ extern "C" int bte_multi_miller_loop(const bte_p1_affine* p, const bte_p2_affine
   if (npairs)
     hipLaunchKernelGGL(k_miller_pairs, dim3(blocks_for(npairs, 64)), dim3(64), 0, s, (const g1a*)p,
                        (const g2a*)q, npairs, f);
+  hipLaunchKernelGGL(k_segment_product, dim3((unsigned)nseg), dim3(64), 0, s, (const fp12*)f, seg_offsets, nseg,
+                     (fp12*)out);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
+// bte_multi_miller_loop with its per-pair Miller values in the file-scope scratch array instead
+// of the stream workspace: the same kernels, no allocation of any kind.  Every pair's value is
+// written before it is read, so nothing needs clearing; calls that overlap on one GPU (several
+// streams or threads) share the array and race.
+extern "C" int bte_multi_miller_loop_noalloc(const bte_p1_affine* p, const bte_p2_affine* q, size_t npairs,
+                                             const uint64_t* seg_offsets, size_t nseg, bte_fp12* out,
+                                             void* stream) {
+  if (npairs > NOALLOC_MAX_PAIRS)
+    return set_error(BTE_EINVAL, "multi_miller_loop_noalloc: %zu pairs exceed the scratch of %zu pairs", npairs,
+                     NOALLOC_MAX_PAIRS);
+  if (nseg == 0) return BTE_OK;
+  if (!seg_offsets || !out || (npairs && (!p || !q)))
+    return set_error(BTE_EINVAL, "multi_miller_loop_noalloc: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  void* f = nullptr;
+  BTE_TRY_RC(scratch_address(&f));
+  if (npairs)
+    hipLaunchKernelGGL(k_miller_pairs, dim3(blocks_for(npairs, 64)), dim3(64), 0, s, (const g1a*)p,
+                       (const g2a*)q, npairs, (fp12*)f);
   hipLaunchKernelGGL(k_segment_product, dim3((unsigned)nseg), dim3(64), 0, s, (const fp12*)f, seg_offsets, nseg,
                      (fp12*)out);
   BTE_CHECK_LAUNCH();
