@@ -138,14 +138,23 @@ def _timed(fn, reps=3):
     return out, best * 1e3
 
 
-def bench_fft_sizes():
+def bench_fft_sizes(with_cpu=True):
     """Radix-2 group FFTs over the Fr domain: G1 2^8/2^12/2^16, G2 2^8/2^12, GT 2^8 (forward), and
     64 G1 transforms of log-uniform sizes 2^6..2^12 in one launch sequence; inputs s_i * G, parity
-    = inverse(forward(x)) == x."""
+    = inverse(forward(x)) == x.  G1 and GT rows up to 2^12 carry the oracle's C transform on the
+    host cores (cpu_ms)."""
     import torch
 
+    import weighted_bte_amd as W
     from weighted_bte_amd import api, inputs
 
+    lib = None
+    if with_cpu:
+        try:
+            from oracle import cpu as ocpu
+            lib = ocpu.load()
+        except ImportError:
+            lib = None
     rows = []
     for group, logn in ((1, 8), (1, 12), (1, 16), (2, 8), (2, 12), (3, 8)):
         n = 1 << logn
@@ -163,8 +172,20 @@ def bench_fft_sizes():
             norm = api.batch_normalize_g1 if group == 1 else api.batch_normalize_g2
         y, ms = _timed(lambda: fwd(x))
         ok = torch.equal(norm(inv(y)), norm(x))
-        rows.append({"group": ("G1", "G2", "GT")[group - 1], "n": n, "forward_ms": round(ms, 3),
-                     "butterflies_per_s": round((n // 2) * logn / (ms * 1e-3), 1), "roundtrip_ok": bool(ok)})
+        row = {"group": ("G1", "G2", "GT")[group - 1], "n": n, "forward_ms": round(ms, 3),
+               "butterflies_per_s": round((n // 2) * logn / (ms * 1e-3), 1), "roundtrip_ok": bool(ok)}
+        if group != 3 and logn <= 8:
+            # small transforms run as one batched MSM of the n x n DFT; the staged radix-2 path
+            # for comparison (same outputs)
+            W.call("bte_fft_set_dft_max_log", 0)
+            y2, ms2 = _timed(lambda: fwd(x))
+            W.call("bte_fft_set_dft_max_log", 8)
+            row.update({"path": "dft_as_msm", "staged_ms": round(ms2, 3),
+                        "paths_agree": bool(torch.equal(norm(y2), norm(y)))})
+        if lib is not None and group != 2 and logn <= 12:
+            row["cpu_ms"] = round(ocpu.time_fft_sample(lib, group, logn) * 1e3, 2)
+            row["cpu_cores"] = int(lib.oracle_max_threads())
+        rows.append(row)
     rng = __import__("random").Random(64)
     logs = [6 + rng.randrange(7) for _ in range(64)]
     total = sum(1 << l for l in logs)
@@ -230,7 +251,7 @@ def bench_multi_pairing(nslots=512, seed=11):
             "pairs_per_s_prepared": round(npairs / (ms_pr * 1e-3), 1), "parity": bool(ok)}
 
 
-def bench_pfe(B=512, n_parties=128, total_weight=4096, batches=128, plain_batches=2):
+def bench_pfe(B=512, n_parties=128, total_weight=4096, batches=128, plain_batches=2, with_cpu=True):
     """weighted_pfe batch decryption at B = 512 (n = 128 Pareto(1.16) parties, W = 4096,
     heaviest-first committee): prepare, then precompute + open per batch through the plain
     per-call entry points and for `batches` batches through the fused ones (d1 line tables
@@ -280,6 +301,14 @@ def bench_pfe(B=512, n_parties=128, total_weight=4096, batches=128, plain_batche
     ph_open = Wm.profile_read()
     Wm.profile_enable(False)
     ok = torch.equal(out_plain, msgs[:plain_batches * B * 576]) and torch.equal(out_fused, msgs)
+    cpu = None
+    if with_cpu:
+        try:
+            from oracle import cpu as ocpu
+            lib = ocpu.load()
+            cpu = ocpu.time_pfe_decrypt_sample(lib, B=B, T=T, B1=64) if lib is not None else None
+        except ImportError:
+            cpu = None
     plain_rate = plain_batches * B / (ms_plain * 1e-3)
     fused_rate = batches * B / (ms_fused * 1e-3)
     return {"workload": "weighted_pfe decrypt", "B": B, "parties": n_parties, "total_weight": total_weight,
@@ -290,6 +319,8 @@ def bench_pfe(B=512, n_parties=128, total_weight=4096, batches=128, plain_batche
             "ctxts_per_s_cold_fused": round(batches * B / ((ms_fused + ms_prep) * 1e-3), 1),
             "phases_ms_fused": {"precompute": {k: round(v, 2) for k, v in ph_pre.items()},
                                 "open": {k: round(v, 2) for k, v in ph_open.items()}},
+            "fused_over_cpu": round(fused_rate / cpu["value"], 2) if cpu else None,
+            "cpu_baseline": cpu,
             "plaintexts_bit_exact": bool(ok)}
 
 
@@ -382,12 +413,49 @@ def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batche
     ph_open1 = W.profile_read()
     W.profile_enable(False)
     n = B * batches
+    # roofline of the two dominant kernels of the fused pass, from their phase times and the exact
+    # operation counts: the cyclotomic prefix inverse FFT (k_cyc_stage, counted from the same
+    # twiddle plans the kernels use) and the hard parts (fixed chain: 315 cyclotomic squarings,
+    # 34 products, 3 Frobenius maps); Fp-mul equivalents: Fp12 product 54, cyclotomic square 18,
+    # Frobenius 15 (five Fp2 products)
+    import ctypes
+    cnt = (ctypes.c_uint64 * 3)()
+    W.call("bte_cyclotomic_prefix_work", 2 * B if (2 * B) & (2 * B - 1) == 0 else 1 << (2 * B - 1).bit_length(),
+           B, cnt)
+    cyc_fp_mul = batches * (18 * cnt[0] + 54 * cnt[1] + 15 * cnt[2])
+    hard_fp_mul = n * (315 * 18 + 34 * 54 + 3 * 15)
+
+    def roof(kernel, fp_mul, ms):
+        ach = fp_mul * MACS_PER_FP_MUL / (ms * 1e-3) / 1e12
+        return {"bound": "valu", "kernel": kernel, "fp_mul": int(fp_mul), "ms": round(ms, 2),
+                "achieved": round(ach, 3), "peak": round(LANE_OPS_PEAK / 1e12, 2),
+                "unit": "TOP/s (u32 multiply-accumulates)", "frac": round(ach / (LANE_OPS_PEAK / 1e12), 4)}
+    roofline = {"cyclotomic_prefix_ifft": roof("k_cyc_stage", cyc_fp_mul, ph_pre["cyclo_ifft"]),
+                "hard_part": roof("k_final_exp_hard", hard_fp_mul, ph_pre["hard"])}
+    # validate and accept (weighted_btx/src/decryption.rs:85-140, :276-422) for one batch of B:
+    # the per-ciphertext Schnorr checks after the host's hashing, and the challenge-folded share
+    # check of the committee (one G1 MSM, one batched indexed G2 MSM, one multi-pairing)
+    rs = workloads.inputs.Stream(5, 7000)
+    kk, ee = rs.scalars(B), rs.scalars(B)
+    r0 = wl.r[:B]
+    commit = api.g1_generator_mul(api.to_device(workloads.inputs.pack_scalars(kk)))
+    resp = [(k + e * r) % R for k, e, r in zip(kk, ee, r0)]
+    ok_v, t_validate = timed(lambda: api.btx_verify_proofs(commit, ct1[:B * 96], resp, ee))
+    vk = api.g2_generator_mul(api.to_device(workloads.inputs.pack_scalars(
+        [pow(q, i + 1, R) for i in range(wl.L) for q in wl.q])))
+    rho = [x or 1 for x in rs.scalars(T)]
+    ok_a, t_accept = timed(lambda: api.btx_verify_shares_batched(shares[:T * 144], committee, rho, vk, wl.N, wl.L,
+                                                                  ct1[:B * 96], valid[:B]))
+    validate_accept = {"B": B, "validate_ms": round(t_validate * 1e3, 2), "validate_all_valid": bool(all(ok_v)),
+                       "accept_ms": round(t_accept * 1e3, 2), "accept_ok": bool(ok_a),
+                       "what": "validate: B Schnorr checks (s G == A + e C) after the host's hashing; accept: "
+                               "batched share check of the committee's T shares"}
     cpu = None
     if with_cpu:
         try:
             from oracle import cpu as ocpu
             lib = ocpu.load()
-            cpu = ocpu.time_btx_decrypt_sample(lib, B=B, T=T) if lib is not None else None
+            cpu = ocpu.time_btx_decrypt_sample(lib, B=B, T=T, B1=64) if lib is not None else None
         except ImportError:
             cpu = None
     fused_rate = n / t_fused
@@ -406,6 +474,8 @@ def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batche
         "ctxts_per_s_cold_fused": round(n / (t_prepare + t_fused), 1),
         "fused_over_plain": round(fused_rate / (plain_batches * B / t_plain), 2),
         "fused_over_cpu": round(fused_rate / cpu["value"], 2) if cpu else None,
+        "roofline": roofline,
+        "validate_accept": validate_accept,
         "phases_ms_fused": {"precompute": {k: round(v, 2) for k, v in ph_pre.items()},
                             "open": {k: round(v, 2) for k, v in ph_open.items()}},
         "phases_ms_plain": {"precompute": {k: round(v, 2) for k, v in ph_pre1.items()},
@@ -415,12 +485,15 @@ def bench_decrypt(batches, B=512, n_parties=128, total_weight=4096, plain_batche
 
 
 def bench_serving(n_requests=48, rate=40.0, burst_requests=256, burst_rate=2000.0, max_batch=256, seed=7,
-                  max_weight=2048, weight_per_party=4):
-    """Poisson arrivals of batch-decryption requests (B log-uniform in 16..max_batch) against
-    three resident committees (n log-uniform in 16..1000, Pareto(1.16) weights summing to
-    W = min(max_weight, weight_per_party * n)), admitted by HBM budget, fused across committees
-    and batch sizes.  Two offered loads: `rate` (latency at moderate load) and `burst_rate` over
-    `burst_requests` requests (throughput at saturation)."""
+                  max_weight=2048, weight_per_party=4, pfe_batch=64, pfe_share=0.25):
+    """Poisson arrivals of batch-decryption requests against three resident weighted_btx
+    committees (n log-uniform in 16..1000, Pareto(1.16) weights summing to
+    W = min(max_weight, weight_per_party * n); B log-uniform in 16..max_batch) and one weighted_pfe
+    committee (n = 64, W = 256, batch size `pfe_batch`; a request of b log-uniform in
+    8..pfe_batch ciphertexts is completed to pfe_batch slots), a `pfe_share` of the requests
+    going to the latter; admitted by HBM budget, fused across committees and batch sizes.  Two
+    offered loads: `rate` (latency at moderate load) and `burst_rate` over `burst_requests`
+    requests (throughput at saturation, with one and with two launches in flight)."""
     import dataclasses
     import math
     import random
@@ -440,8 +513,22 @@ def bench_serving(n_requests=48, rate=40.0, burst_requests=256, burst_rate=2000.
         committee = workloads.heaviest_first(weights, wl.t)
         wls.append((wl, committee))
         coms.append(serving.Committee(wl.material, wl.W, wl.L, wl.offsets, committee))
+    pw = workloads.pareto_weights(64, 256, seed=seed + 3)
+    pfe = workloads.PfeWorkload(pw, 128, pfe_batch, seed=seed + 30)
+    pcom_ids = workloads.heaviest_first(pw, pfe.t)
+    coms.append(serving.PfeCommittee(pfe.material, pfe.W, pfe_batch, pfe.offsets, pcom_ids, pfe.alpha,
+                                     pfe.global_key))
     reqs, expected = [], []
     for rid in range(max(n_requests, burst_requests)):
+        if rnd.random() < pfe_share:
+            b = int(round(math.exp(rnd.uniform(math.log(8), math.log(pfe_batch)))))
+            ct1, ct2, msgs = pfe.encrypt()
+            c1, c2 = serving.pad_pfe_batch(ct1[:96 * b], ct2[:576 * b], pfe_batch)
+            sigma = api.pfe_partial_decrypt(c1, pfe.g)
+            shares = torch.cat([sigma[144 * j:144 * (j + 1)] for j in pcom_ids])
+            reqs.append(serving.Request(rid, 3, b, c1, c2, shares))
+            expected.append(msgs[:576 * b])
+            continue
         cid = rnd.randrange(3)
         B = int(round(math.exp(rnd.uniform(math.log(16), math.log(max_batch)))))
         wl, committee = wls[cid]
@@ -450,7 +537,8 @@ def bench_serving(n_requests=48, rate=40.0, burst_requests=256, burst_rate=2000.
         shares = torch.cat([sig[144 * j:144 * (j + 1)] for j in committee])
         reqs.append(serving.Request(rid, cid, B, ct1, ct2, shares))
         expected.append(msgs)
-    for com in coms:  # committee preparation happens before traffic (resident state)
+    coms[3].prepare()
+    for com in coms[:3]:  # committee preparation happens before traffic (resident state)
         bp = 16
         while bp <= max_batch:
             com.prepare(bp)
@@ -458,25 +546,30 @@ def bench_serving(n_requests=48, rate=40.0, burst_requests=256, burst_rate=2000.
     torch.cuda.synchronize()
     t_setup = time.time() - t0
 
-    def run(nreq, r):
+    def run(nreq, r, inflight=2):
         batch = [dataclasses.replace(q) for q in reqs[:nreq]]
-        srv = serving.BtxServer(coms, hbm_budget=8 << 30, max_fuse=256)
+        srv = serving.BtxServer(coms, hbm_budget=8 << 30, max_fuse=256, max_inflight=inflight)
         p50, p99, rate_ct, wall = serving.run_poisson(srv, batch, r)
-        ok = all(torch.equal(q.messages, e) for q, e in zip(batch, expected))
-        return {"requests": nreq, "offered_rate_req_per_s": r, "ciphertexts": sum(q.batch for q in batch),
+        ok = all(api.to_host(q.messages) == api.to_host(e) for q, e in zip(batch, expected))
+        return {"requests": nreq, "pfe_requests": sum(q.committee == 3 for q in batch),
+                "offered_rate_req_per_s": r, "max_inflight": inflight,
+                "ciphertexts": sum(q.batch for q in batch),
                 "launches": srv.launches, "plaintexts_bit_exact": bool(ok),
                 "p50_latency_ms": round(p50 * 1e3, 1), "p99_latency_ms": round(p99 * 1e3, 1),
                 "ctxts_per_s": round(rate_ct, 1), "wall_s": round(wall, 2)}
 
     moderate = run(n_requests, rate)
-    burst = run(burst_requests, burst_rate) if burst_requests else None
-    out = {"workload": "weighted_btx serving (Poisson arrivals, continuous batching)",
+    run(min(32, burst_requests), burst_rate, 1)  # warm-up of the one-stream server's workspace
+    burst1 = run(burst_requests, burst_rate, 1) if burst_requests else None
+    burst = run(burst_requests, burst_rate, 2) if burst_requests else None
+    out = {"workload": "weighted_btx + weighted_pfe serving (Poisson arrivals, continuous batching)",
            "max_weight": max_weight,
            "committees": len(coms), "committee_sizes": [len(c.accepted) for c in coms],
-           "total_weights": [c.total_weight for c in coms], "max_batch": max_batch,
-           "setup_s": round(t_setup, 1), "saturated": burst}
+           "total_weights": [c.total_weight for c in coms], "max_batch": max_batch, "pfe_batch": pfe_batch,
+           "setup_s": round(t_setup, 1), "saturated": burst, "saturated_one_in_flight": burst1,
+           "overlap_gain": round(burst["ctxts_per_s"] / burst1["ctxts_per_s"], 3) if burst else None}
     out.update(moderate)
-    out["plaintexts_bit_exact"] = moderate["plaintexts_bit_exact"] and (burst is None or burst["plaintexts_bit_exact"])
+    out["plaintexts_bit_exact"] = all(x is None or x["plaintexts_bit_exact"] for x in (moderate, burst, burst1))
     return out
 
 
@@ -578,9 +671,9 @@ def main():
     msm_fp_mul = nw * (11 * n + 32 * (1 << c)) + 7 * 255
     mac_peak = W.probe_mac_rate(torch.cuda.current_stream().cuda_stream)
     msm_sweep = None if a.no_sizes else bench_msm_sizes(not a.no_tune)
-    ffts = None if a.no_sizes else bench_fft_sizes()
+    ffts = None if a.no_sizes else bench_fft_sizes(with_cpu=not a.no_cpu)
     mpair = None if a.no_sizes else bench_multi_pairing()
-    pfe = None if a.no_pfe else bench_pfe()
+    pfe = None if a.no_pfe else bench_pfe(with_cpu=not a.no_cpu)
 
     cpu = None if a.no_cpu else cpu_baseline(a.logn)
     decrypt = bench_decrypt(a.decrypt_batches, with_cpu=not a.no_cpu) if a.decrypt_batches > 0 else None

@@ -118,6 +118,12 @@ int bte_g2_decompress(const uint8_t* in, size_t n, bte_p2_affine* points, uint8_
 int bte_fft_fr(uint8_t* values, size_t n, int inverse, int scaled, void* stream);
 int bte_fft_fr_batched(uint8_t* values, size_t n, size_t count, int inverse, int scaled, void* stream);
 
+/* G1/G2 transforms of size 2^2 .. 2^log_n (default 8) with n^2 x count <= 2^20 run as one
+ * batched MSM per call (X_k = sum_j w^(jk) x_j, one segment per output) instead of log n
+ * latency-bound stages; 0 turns that off.  The outputs are the same points.  No counterpart in
+ * the reference (its transform is always staged); set by the autotuner. */
+int bte_fft_set_dft_max_log(int log_n);
+
 /* weighted_btx/src/fft.rs:197-234 Radix2Domain::{fft, ifft, ifft_unscaled} over G1Projective */
 int bte_fft_g1(bte_p1* values, size_t n, int inverse, int scaled, void* stream);
 /* same over G2Projective */
@@ -389,6 +395,22 @@ int bte_pfe_open_prepared_batched(const bte_p1* shares, size_t n_accepted, const
                                   const uint8_t* alpha, size_t batch_size, size_t count,
                                   bte_fp12* messages_out, void* stream);
 
+/* Plan of the fused weighted_pfe step (bte_pfe_precompute_batched then
+ * bte_pfe_open_prepared_batched for `count` batches of one committee, same outputs): creation
+ * keeps the committee's line tables and the B alpha points (HOST, canonical) and allocates the
+ * Cauchy intermediates; bte_pfe_plan_launch_graph captures one run into a HIP graph on its first
+ * call (non-null stream; later calls must pass the same buffers) and replays it.  No counterpart
+ * in the reference (its steps are rayon loops). */
+typedef struct bte_pfe_plan bte_pfe_plan;
+int bte_pfe_plan_create(size_t batch_size, size_t count, size_t n_accepted, const bte_fp6* total_lines,
+                        const bte_fp6* d1_lines, const bte_fp6* mask_lines, const uint8_t* alpha,
+                        bte_pfe_plan** plan, void* stream);
+int bte_pfe_plan_run(bte_pfe_plan* plan, const bte_p1_affine* ct1, const bte_p1* shares, const bte_fp12* ct2,
+                     bte_fp12* messages_out, void* stream);
+int bte_pfe_plan_launch_graph(bte_pfe_plan* plan, const bte_p1_affine* ct1, const bte_p1* shares,
+                              const bte_fp12* ct2, bte_fp12* messages_out, void* stream);
+int bte_pfe_plan_destroy(bte_pfe_plan* plan);
+
 /* weighted_pfe/src/proof.rs:74-122 verify_batch after the host-side transcript hashing: 1 when
  * sum c_i s_i G - sum c_i A_i - sum c_i e_i C_i is the identity (one G1 MSM of 2B+1 terms).
  * commitments A and ct1 C: B G1 affine (device); responses s, challenges e, coefficients c: B
@@ -415,6 +437,10 @@ int bte_g2_generator_mul(const uint8_t* scalars, size_t n, bte_p2_affine* out, v
  * No counterpart in the reference (measurement support).
  * ---------------------------------------------------------------------------------------- */
 int bte_profile_enable(int on);
+/* Operations of one unscaled prefix inverse cyclotomic transform of size n with output_count
+ * outputs, counted from the twiddle plans the kernels use (HOST only): counts[0] cyclotomic
+ * squarings, counts[1] Fp12 products, counts[2] Frobenius maps. */
+int bte_cyclotomic_prefix_work(size_t n, size_t output_count, uint64_t* counts);
 /* "phase=ms;phase=ms..." of the most recent profiled call. */
 int bte_profile_read(char* buf, size_t len);
 /* Measured 32x32->64 multiply-accumulates per second of the whole device. */

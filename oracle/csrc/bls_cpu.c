@@ -645,3 +645,16 @@ int oracle_max_threads(void) {
   return 1;
 #endif
 }
+
+/* Thread count of the OpenMP regions above (0: all processors), for the one-thread rows of the
+ * CPU baseline. */
+int oracle_set_threads(int n) {
+#ifdef _OPENMP
+  extern void omp_set_num_threads(int);
+  extern int omp_get_num_procs(void);
+  omp_set_num_threads(n > 0 ? n : omp_get_num_procs());
+#else
+  (void)n;
+#endif
+  return 0;
+}

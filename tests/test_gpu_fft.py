@@ -277,3 +277,59 @@ def test_cyclotomic_prefix_one_lane_equals_three_lane(api):
     a = a.view(count, n * 576)[:, :outs * 576]
     b = b.view(count, n * 576)[:, :outs * 576]
     assert torch.equal(a, b)
+
+
+def _jac_from_affine_device(api, aff, group):
+    import torch
+
+    w = 96 if group == 1 else 192
+    n = aff.numel() // w
+    one = ((1 << 384) % O.P).to_bytes(48, "little") + (bytes(48) if group == 2 else b"")
+    rows = aff.view(n, w)
+    z = api.to_device(one).view(1, -1).repeat(n, 1)
+    z[(rows == 0).all(dim=1)] = 0  # the all-zero affine encoding is the identity: Z = 0
+    return torch.cat([rows, z], dim=1).reshape(-1).contiguous()
+
+
+@pytest.mark.parametrize("group", [1, 2])
+def test_small_transforms_by_msm_equal_staged_and_closed_form(api, group):
+    """Transforms up to 2^8 run as one batched MSM (X_k = sum_j w^(jk) x_j): the same points as
+    the staged radix-2 transform for forward, inverse and scaled inverse, single and batched,
+    with an identity input; and the closed form DFT(s) * G at 2^8."""
+    import ctypes
+
+    import torch
+
+    from weighted_bte_amd import call
+
+    rnd = random.Random(80 + group)
+    gen = api.g1_generator_mul if group == 1 else api.g2_generator_mul
+    norm = api.batch_normalize_g1 if group == 1 else api.batch_normalize_g2
+    fn = "bte_fft_g1_batched" if group == 1 else "bte_fft_g2_batched"
+    width = 144 if group == 1 else 288
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for logn, count in ((2, 1), (6, 3), (8, 1)):
+        n = 1 << logn
+        s = [rnd.randrange(O.R) for _ in range(n * count)]
+        s[1] = 0  # an identity input
+        x = _jac_from_affine_device(api, gen(api.to_device(b"".join(v.to_bytes(32, "little") for v in s))), group)
+        for inverse, scaled in ((0, 0), (1, 1), (1, 0)):
+            outs = []
+            for max_log in (8, 0):
+                call("bte_fft_set_dft_max_log", max_log)
+                y = x.clone()
+                call(fn, ctypes.c_void_p(y.data_ptr()), n, count, inverse, scaled, st)
+                outs.append(norm(y))
+            call("bte_fft_set_dft_max_log", 8)
+            assert torch.equal(outs[0], outs[1]), (logn, count, inverse, scaled)
+            assert outs[0].numel() == n * count * width // 3 * 2
+        if logn == 8:
+            y = x.clone()
+            call(fn, ctypes.c_void_p(y.data_ptr()), n, 1, 0, 0, st)
+            got = api.to_host(norm(y))
+            want = O.fr_fft(s)
+            aw = 96 if group == 1 else 192
+            for k in (0, 1, 77, 255):
+                pt = (O.g1_affine_from_bytes if group == 1 else O.g2_affine_from_bytes)(got[aw * k:aw * (k + 1)])
+                mul = g1_mul_gen if group == 1 else g2_mul_gen
+                assert pt == (mul(want[k]) if want[k] else None), "closed form k=%d" % k

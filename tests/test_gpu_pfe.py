@@ -124,3 +124,42 @@ def test_pfe_gpu_keygen_matches_host(api):
     for j in range(N):
         for i in range(B):
             assert g[j][i] * (wl.rho[j] + wl.alpha[i]) % O.R == 1
+
+
+@pytest.mark.parametrize("B,K", [(8, 3), (64, 4)])
+def test_pfe_plan_run_and_graph_replays(api, B, K):
+    """The weighted_pfe plan gives the fused entry points' messages; its HIP graph, replayed three
+    times with eager work and synchronisation between replays, gives them every time."""
+    import torch
+
+    from weighted_bte_amd import workloads
+
+    weights = [2, 3, 1, 4]
+    wl = workloads.PfeWorkload(weights, threshold=5, batch_size=B, seed=90 + B)
+    committee = workloads.heaviest_first(weights, wl.t)
+    T = len(committee)
+    ct1, ct2, msgs = wl.encrypt(K)
+    sh = []
+    for b in range(K):
+        sigma = api.pfe_partial_decrypt(ct1[b * B * 96:(b + 1) * B * 96], wl.g)
+        sh.append(torch.cat([sigma[144 * j:144 * (j + 1)] for j in committee]))
+    shares = torch.cat(sh)
+    d1p, tl, ml = api.pfe_prepare(wl.material, wl.W, B, wl.offsets, committee, wl.alpha, wl.global_key)
+    d1l = api.prepare_g2_lines(d1p)
+    fc, pc = api.pfe_precompute_batched(ct1, tl, B)
+    want = api.to_host(api.pfe_open_prepared_batched(shares, T, d1l, ml, ct1, ct2, fc, pc, wl.alpha, B))
+    assert want == api.to_host(msgs)
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        plan = api.PfePlan(B, K, T, tl, d1l, ml, wl.alpha)
+        assert api.to_host(plan.run(ct1, shares, ct2)) == want
+        out = torch.empty(B * K * 576, dtype=torch.uint8, device="cuda")
+        for rep in range(3):
+            out.zero_()
+            plan.launch_graph(ct1, shares, ct2, out)
+            stream.synchronize()
+            assert api.to_host(out) == want, "replay %d" % rep
+            api.pfe_precompute_batched(ct1, tl, B)  # eager work between replays
+            torch.cuda.synchronize()
+        plan.close()

@@ -58,7 +58,9 @@ def test_serving_alone_equals_mixed(dev):
 
 
 def test_decrypt_requests_two_in_flight(dev):
-    """Two fused launches on two streams overlap; each result equals the plain path."""
+    """With max_fuse = 1 every request is its own launch, at most two in flight on two streams;
+    each result equals the plain path.  (That the two streams' kernels overlap on the device is
+    measured by the bench: throughput at max_inflight 1 against 2, and a kernel trace.)"""
     from weighted_bte_amd import api, serving, workloads
 
     weights = [1, 2, 2, 1]
@@ -97,3 +99,42 @@ def test_serving_accepts_any_batch_up_to_max(dev):
         assert api.to_host(srv.decrypt_alone(r)) == want
     with pytest.raises(ValueError):
         srv.submit(serving.Request(99, 0, 101, reqs[0].ct1, reqs[0].ct2, reqs[0].shares))
+
+
+def test_serving_pfe_and_btx_requests_alone_equal_mixed(dev):
+    """weighted_pfe requests (completed to the committee's B = 16 slots) and weighted_btx
+    requests in one server: every request's plaintexts are the encrypted messages, alone (plain
+    per-call path) and in the fused mix, which runs all pfe requests of a committee in one fused
+    precompute + open."""
+    import torch
+
+    from weighted_bte_amd import api, serving, workloads
+
+    B = 16
+    pw = [2, 1, 3, 2]
+    pfe = workloads.PfeWorkload(pw, threshold=4, batch_size=B, seed=23)
+    pcom_ids = workloads.heaviest_first(pw, pfe.t)
+    pcom = serving.PfeCommittee(pfe.material, pfe.W, B, pfe.offsets, pcom_ids, pfe.alpha, pfe.global_key)
+    bw = [1, 2, 2, 1]
+    btx = workloads.BtxWorkload(bw, threshold=3, max_batch=16, seed=24)
+    bcom = serving.Committee(btx.material, btx.W, btx.L, btx.offsets, workloads.heaviest_first(bw, btx.t))
+    reqs, expected = [], []
+    for rid, b in enumerate((16, 5, 11, 1)):
+        ct1, ct2, msgs = pfe.encrypt()
+        c1, c2 = serving.pad_pfe_batch(ct1[:96 * b], ct2[:576 * b], B)
+        sigma = api.pfe_partial_decrypt(c1, pfe.g)
+        shares = torch.cat([sigma[144 * j:144 * (j + 1)] for j in pcom_ids])
+        reqs.append(serving.Request(rid, 0, b, c1, c2, shares))
+        expected.append(api.to_host(msgs)[:576 * b])
+    breqs, bexp = make_requests(api, workloads, serving, [(btx, bcom.accepted)], [(0, 7), (0, 16)])
+    for r in breqs:
+        r.committee = 1
+        r.rid += 10
+    srv = serving.BtxServer([pcom, bcom], max_fuse=8)
+    for r in reqs + breqs:
+        srv.submit(r)
+    srv.drain()
+    for r, want in zip(reqs + breqs, expected + bexp):
+        got = api.to_host(r.messages)
+        assert got == want, "request %d (mixed)" % r.rid
+        assert api.to_host(srv.decrypt_alone(r)) == want, "request %d (alone)" % r.rid

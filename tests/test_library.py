@@ -45,3 +45,43 @@ def test_error_path_without_gpu_work():
     rc = lib.bte_msm_set_window(7, 16, 3)
     assert rc == 1
     assert b"bad arguments" in lib.bte_last_error()
+
+
+def test_cyclotomic_prefix_work_counter_matches_oracle_plans():
+    """The library's count of the prefix inverse cyclotomic transform's operations (used for the
+    decrypt roofline) equals the count from the oracle's own plans of the same twiddles
+    (fft.rs:33-154 decomposition and width-4 NAF), for the bench shape and a small one.  Host-only
+    call: no GPU needed."""
+    import weighted_bte_amd
+    from helpers import O
+
+    lib = weighted_bte_amd.lib()
+    for n, out in ((16, 5), (1024, 512)):
+        counts = (ctypes.c_uint64 * 3)()
+        assert lib.bte_cyclotomic_prefix_work(n, out, counts) == 0
+        winv = O.Radix2Domain(n).generator_inverse
+        logn = n.bit_length() - 1
+        sqr = mul = frob = 0
+
+        def pow_work(s):
+            nonlocal sqr, mul, frob
+            nafs, count = O.cyclotomic_scalar_plan(s)
+            if count == 0:
+                return
+            nnz = sum(1 for d in nafs for x in d if x)
+            sqr += count
+            mul += 3 + nnz - 1
+            frob += 12
+
+        for lh in range(logn - 1):
+            half = 1 << lh
+            for k in range(half):
+                for _ in range(n // (2 * half)):
+                    if k:
+                        pow_work(pow(winv, k << (logn - 1 - lh), O.R))
+                    mul += 2
+        for k in range(out):
+            if k:
+                pow_work(pow(winv, k, O.R))
+            mul += 1
+        assert list(counts) == [sqr, mul, frob], n

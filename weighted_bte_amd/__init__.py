@@ -39,6 +39,7 @@ _SIGNATURES = {
     "bte_g1_generator_mul": [_vp, _sz, _vp, _vp],
     "bte_g2_generator_mul": [_vp, _sz, _vp, _vp],
     "bte_fft_g1": [_vp, _sz, _i, _i, _vp],
+    "bte_fft_set_dft_max_log": [_i],
     "bte_fft_g2": [_vp, _sz, _i, _i, _vp],
     "bte_fft_gt": [_vp, _sz, _i, _i, _vp],
     "bte_fft_cyclotomic": [_vp, _sz, _i, _vp],
@@ -83,6 +84,10 @@ _SIGNATURES = {
     "bte_btx_plan_run": [_vp, _vp, _vp, _vp, _vp, _vp],
     "bte_btx_plan_launch_graph": [_vp, _vp, _vp, _vp, _vp, _vp],
     "bte_btx_plan_destroy": [_vp],
+    "bte_pfe_plan_create": [_sz, _sz, _sz, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp), _vp],
+    "bte_pfe_plan_run": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "bte_pfe_plan_launch_graph": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "bte_pfe_plan_destroy": [_vp],
     "bte_btx_keygen": [_vp, _sz, _vp, _vp, _sz, _sz, _vp, _vp, _vp],
     "bte_btx_verify_shares": [_vp, _vp, _sz, _vp, _sz, _sz, _vp, _vp, _sz, _vp, _vp],
     "bte_pfe_verify_proofs_batched": [_vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp],
@@ -96,6 +101,7 @@ _SIGNATURES = {
     "bte_pfe_open_prepared_batched": [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _sz, _vp, _vp],
     "bte_multi_miller_loop_prepared": [_vp, _vp, _sz, _vp, _sz, _vp, _vp],
     "bte_profile_enable": [_i],
+    "bte_cyclotomic_prefix_work": [_sz, _sz, _vp],
     "bte_profile_read": [ctypes.c_char_p, _sz],
     "bte_probe_mac_rate": [ctypes.POINTER(ctypes.c_double), _vp],
     "bte_device_alloc": [ctypes.POINTER(_vp), _sz],

@@ -740,3 +740,41 @@ def pfe_partial_decrypt(ct1_affine, fractions):
     out = _empty(N * G1_JAC, ct1)
     call("bte_pfe_partial_decrypt", _ptr(ct1), B, _ptr(fr), N, _ptr(out), _stream())
     return out
+
+
+class PfePlan:
+    """Plan of the fused weighted_pfe step for `count` batches of one committee (total_lines /
+    mask_lines from pfe_prepare, d1_lines = prepare_g2_lines(d1_by_party), alpha the B alpha
+    points): run() is pfe_precompute_batched + pfe_open_prepared_batched, launch_graph() replays a
+    HIP graph of one run (bound to the buffers of its first call; not on the null stream)."""
+
+    def __init__(self, batch_size, count, n_accepted, total_lines, d1_lines, mask_lines, alpha):
+        self._keep = (_dev(total_lines), _dev(d1_lines), _dev(mask_lines))
+        self.B, self.K, self.T = batch_size, count, n_accepted
+        h = ctypes.c_void_p(0)
+        call("bte_pfe_plan_create", batch_size, count, n_accepted, _ptr(self._keep[0]), _ptr(self._keep[1]),
+             _ptr(self._keep[2]), _host_scalars(alpha), ctypes.byref(h), _stream())
+        self.handle = h
+
+    def _args(self, ct1, shares, ct2, out):
+        return (self.handle, _ptr(_dev(ct1)), _ptr(_dev(shares)), _ptr(_dev(ct2)), _ptr(out), _stream())
+
+    def run(self, ct1, shares, ct2, out=None):
+        out = out if out is not None else _empty(self.B * self.K * FP12, ct2)
+        call("bte_pfe_plan_run", *self._args(ct1, shares, ct2, out))
+        return out
+
+    def launch_graph(self, ct1, shares, ct2, out):
+        call("bte_pfe_plan_launch_graph", *self._args(ct1, shares, ct2, out))
+        return out
+
+    def close(self):
+        if self.handle:
+            call("bte_pfe_plan_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -18,6 +18,7 @@
 // whenever a stage has at least 64 butterflies per twiddle, instead of 64 different exponents
 // whose union of add steps every lane would wait through.
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -28,6 +29,11 @@
 #include "coop12.hpp"
 #include "cyclo.hpp"
 #include "host_field.hpp"
+
+extern "C" {
+int bte_g1_batch_normalize(const bte_p1* in, size_t n, bte_p1_affine* out, void* stream);
+int bte_g2_batch_normalize(const bte_p2* in, size_t n, bte_p2_affine* out, void* stream);
+}
 
 namespace bte {
 
@@ -420,6 +426,94 @@ __global__ void __launch_bounds__(64) k_cyc_pow_plans(fp12* __restrict__ v, size
 // ---------------------------------------------------------------------------------------------
 // Drivers
 // ---------------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------------
+// Small group transforms as one batched MSM.  A stage of the radix-2 transform is one thread per
+// butterfly running a whole scalar multiplication (a serial chain of ~128 doublings), so a size-n
+// transform costs log n such chains end to end whatever the GPU's width: 42 ms for G1 at 2^8.
+// Written as n^2 scalar products X_k = sum_j (w^(jk)) x_j, one MSM segment per output, the same
+// transform is one pass of the batched Pippenger kernels, whose serial part is one window
+// combination per output: more arithmetic (n / log n times the butterflies' multiplications) but
+// in parallel, so it wins while n^2 count stays small.  The outputs are the same points (the
+// transform is a linear map); their Jacobian coordinates may differ from the staged form.
+// ---------------------------------------------------------------------------------------------
+struct DftScalars {
+  uint8_t* w = nullptr;  // n canonical scalars (w^j, w^-j, or w^-j n^-1), 32 bytes each
+};
+static std::mutex g_dft_mu;
+static std::map<std::tuple<int, int, int>, DftScalars> g_dft;  // (device, logn, 0 fwd / 1 inv / 2 inv scaled)
+static std::atomic<int> g_dft_max_log{8};                      // sizes up to 2^8 (bte_fft_set_dft_max_log)
+static constexpr size_t DFT_MAX_TERMS = (size_t)1 << 20;
+
+static int dft_scalars(int logn, int kind, const uint8_t** out) {
+  int dev = 0;
+  BTE_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_dft_mu);
+  DftScalars& t = g_dft[std::make_tuple(dev, logn, kind)];
+  if (!t.w) {
+    const size_t n = (size_t)1 << logn;
+    const bteh::Fr w = domain_generator(logn, kind != 0);
+    bteh::Fr cur = kind == 2 ? bteh::fr_inv(bteh::fr_from_u64(n)) : bteh::fr_from_u64(1);
+    std::vector<bteh::Fr> v(n);
+    for (size_t j = 0; j < n; ++j) {
+      v[j] = cur;
+      cur = bteh::fr_mul(cur, w);
+    }
+    uint8_t* d = nullptr;
+    const hipError_t e = upload_new(&d, (const uint8_t*)v.data(), 32 * n);
+    if (e != hipSuccess) return set_error(BTE_ENOMEM, "dft scalars (2^%d): %s", logn, hipGetErrorString(e));
+    t.w = d;
+  }
+  *out = t.w;
+  return BTE_OK;
+}
+
+// Term t = (b n + k) n + j of `count` transforms: point b n + j with scalar index j k mod n, in
+// segment b n + k; seg_off[s] = s n.
+__global__ void k_dft_terms(int logn, size_t count, uint32_t* __restrict__ pidx, uint32_t* __restrict__ sidx,
+                            uint64_t* __restrict__ seg_off) {
+  const size_t n = (size_t)1 << logn;
+  const size_t total = n * n * count;
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t > total) return;
+  if (t == total) {
+    seg_off[n * count] = total;
+    return;
+  }
+  const size_t seg = t >> logn, j = t & (n - 1), k = seg & (n - 1), b = seg >> logn;
+  pidx[t] = (uint32_t)(b * n + j);
+  sidx[t] = (uint32_t)((j * k) & (n - 1));
+  if (j == 0) seg_off[seg] = t;
+}
+
+template <class F> static bool dft_by_msm(int logn, size_t count) {
+  const size_t n = (size_t)1 << logn;
+  return logn >= 2 && logn <= g_dft_max_log && n * n * count <= DFT_MAX_TERMS;
+}
+
+template <class F>
+static int group_dft_msm(jac<F>* v, int logn, size_t count, bool inverse, bool scaled, hipStream_t s) {
+  const size_t n = (size_t)1 << logn, N = n * count, terms = n * N;
+  CallScope sc(s);
+  aff<F>* A;
+  uint32_t *pidx, *sidx;
+  uint64_t* off;
+  const uint8_t* w;
+  BTE_TRY_RC(dft_scalars(logn, inverse ? (scaled ? 2 : 1) : 0, &w));
+  BTE_TRY_RC(sc.dev(&A, N));
+  BTE_TRY_RC(sc.dev(&pidx, terms));
+  BTE_TRY_RC(sc.dev(&sidx, terms));
+  BTE_TRY_RC(sc.dev(&off, N + 1));
+  if (sizeof(F) == sizeof(fp))
+    BTE_TRY_RC(bte_g1_batch_normalize((const bte_p1*)v, N, (bte_p1_affine*)A, s));
+  else
+    BTE_TRY_RC(bte_g2_batch_normalize((const bte_p2*)v, N, (bte_p2_affine*)A, s));
+  hipLaunchKernelGGL(k_dft_terms, dim3(blocks_for(terms + 1, 256)), dim3(256), 0, s, logn, count, pidx, sidx, off);
+  BTE_CHECK_LAUNCH();
+  if (sizeof(F) == sizeof(fp))
+    return g1_msm_batched_gather(A, N, pidx, w, sidx, terms, off, N, 0, v, s);
+  return g2_msm_batched_gather(A, N, pidx, w, sidx, terms, off, N, 0, v, s);
+}
+
 template <class F>
 static int group_fft(jac<F>* v, size_t n, size_t count, bool inverse, bool scaled, hipStream_t s) {
   const int logn = log2_exact(n);
@@ -427,6 +521,7 @@ static int group_fft(jac<F>* v, size_t n, size_t count, bool inverse, bool scale
   if (count == 0) return BTE_OK;
   if (!v) return set_error(BTE_EINVAL, "fft: null values");
   if (n == 1) return BTE_OK;  // a one-point transform is the identity map (n^-1 = 1)
+  if (dft_by_msm<F>(logn, count)) return group_dft_msm<F>(v, logn, count, inverse, scaled, s);
   DomainTables* t;
   int rc = domain_tables(logn, inverse, false, &t, s);
   if (rc) return rc;
@@ -775,6 +870,57 @@ static int upload_plans(CallScope& sc, const uint8_t* host_scalars, size_t n, cy
 }  // namespace bte
 
 using namespace bte;
+
+// Work of one unscaled prefix inverse cyclotomic transform of size n keeping output_count outputs
+// (fft.rs:284-350), from the same twiddle plans the kernels run: per butterfly with a nonzero
+// twiddle, cyclo_pow_plan's table (one cyclotomic squaring, three products, twelve Frobenius
+// maps) and chain (count - 1 squarings, nonzero digits - 1 products), then the butterfly's
+// products (two per butterfly of the full stages, one per output of the last).  Host only.
+extern "C" int bte_cyclotomic_prefix_work(size_t n, size_t output_count, uint64_t* counts) {
+  const int logn = log2_exact(n);
+  if (logn < 1 || !counts || output_count > n / 2) return set_error(BTE_EINVAL, "cyclotomic_prefix_work: bad shape");
+  const bteh::Fr w = domain_generator(logn, true);
+  std::vector<cyc_plan> plans(n / 2);
+  bteh::Fr cur = bteh::fr_from_u64(1);
+  for (size_t j = 0; j < n / 2; ++j) {
+    plans[j] = make_plan(cur);
+    cur = bteh::fr_mul(cur, w);
+  }
+  uint64_t sqr = 0, mul = 0, frob = 0;
+  auto pow_work = [&](const cyc_plan& p) {
+    if (p.count == 0) return;
+    int nnz = 0;
+    for (int c = 0; c < 4; ++c)
+      for (int i = 0; i < CYC_DIGITS; ++i) nnz += p.naf[c][i] != 0;
+    sqr += 1 + (uint64_t)(p.count - 1);
+    mul += 3 + (uint64_t)(nnz - 1);
+    frob += 12;
+  };
+  for (int lh = 0; lh < logn - 1; ++lh) {  // full stages
+    const size_t half = (size_t)1 << lh;
+    for (size_t k = 0; k < half; ++k) {
+      const size_t reps = n / (2 * half);  // butterflies sharing twiddle k
+      for (size_t r = 0; r < reps; ++r) {
+        if (k) pow_work(plans[k << (logn - 1 - lh)]);
+        mul += 2;
+      }
+    }
+  }
+  for (size_t k = 0; k < output_count; ++k) {  // last stage, outputs k < output_count
+    if (k) pow_work(plans[k]);
+    mul += 1;
+  }
+  counts[0] = sqr;
+  counts[1] = mul;
+  counts[2] = frob;
+  return BTE_OK;
+}
+
+extern "C" int bte_fft_set_dft_max_log(int log_n) {
+  if (log_n < 0 || log_n > 10) return set_error(BTE_EINVAL, "fft_set_dft_max_log: %d outside 0..10", log_n);
+  g_dft_max_log.store(log_n);
+  return BTE_OK;
+}
 
 extern "C" int bte_fft_g1(bte_p1* values, size_t n, int inverse, int scaled, void* stream) {
   return group_fft<fp>((jac<fp>*)values, n, 1, inverse != 0, scaled != 0, (hipStream_t)stream);

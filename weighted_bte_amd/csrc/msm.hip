@@ -299,7 +299,8 @@ static constexpr int ACC_BLOCK = 64;
 
 template <class F>
 __global__ void __launch_bounds__(ACC_BLOCK, MsmTraits<F>::ACC_WAVES)
-    k_msm_accumulate(const aff<F>* __restrict__ points, const uint32_t* __restrict__ index, size_t n_points,
+    k_msm_accumulate(const aff<F>* __restrict__ points, uint32_t stride, const uint32_t* __restrict__ index,
+                     size_t n_points,
                      const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ offsets, uint32_t nb,
                      uint32_t tstar, jac<F>* __restrict__ buckets, jac<F>* __restrict__ heads,
                      jac<F>* __restrict__ tails, int* __restrict__ err) {
@@ -344,7 +345,7 @@ __global__ void __launch_bounds__(ACC_BLOCK, MsmTraits<F>::ACC_WAVES)
           continue;
         }
       }
-      aff<F> q = points[idx];
+      aff<F> q = *(const aff<F>*)((const char*)points + (size_t)idx * stride);
       if (e >> 31) q = aff_neg(q);
       acc = jac_add_aff(acc, q);
     }
@@ -355,6 +356,17 @@ __global__ void __launch_bounds__(ACC_BLOCK, MsmTraits<F>::ACC_WAVES)
   }
   __syncthreads();  // the workgroup's head pieces are written
   if (pending) buckets[b] = jac_add(acc, heads[t + 1]);
+}
+
+// Point records padded to a whole number of 128-byte lines (G1: 96 -> 128 bytes, G2: 192 -> 256),
+// so each gather of the accumulation reads one or two full lines instead of straddling three.
+template <class F> struct PadStride {
+  static constexpr uint32_t value = (uint32_t)((sizeof(aff<F>) + 127) / 128 * 128);
+};
+template <class F>
+__global__ void k_msm_pad_points(const aff<F>* __restrict__ in, size_t n, char* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) *(aff<F>*)(out + i * PadStride<F>::value) = in[i];
 }
 
 // Appends the threads with `want` set to a device queue with one atomic per workgroup of T
@@ -786,6 +798,16 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
   BTE_TRY_RC(sc.dev(&nbig, 1));
   BTE_TRY_RC(sc.dev(&mid, nb));
   BTE_TRY_RC(sc.dev(&nmid, 1));
+  const aff<F>* acc_points = points;
+  uint32_t stride = (uint32_t)sizeof(aff<F>);
+  if (!index) {  // contiguous bases: gather from line-aligned copies
+    char* padded;
+    BTE_TRY_RC(sc.dev(&padded, n * (size_t)PadStride<F>::value));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_pad_points<F>), dim3(blocks_for(n, 256)), dim3(256), 0, s, points, n,
+                       padded);
+    acc_points = (const aff<F>*)padded;
+    stride = PadStride<F>::value;
+  }
   HornerJob<HF>* job = nullptr;
   if (!sc.capturing()) {
     BTE_TRY_RC(sc.host(&job, 1));
@@ -827,7 +849,7 @@ static int msm_run(int curve, const aff<F>* points, size_t n_points, const uint3
                        (const int32_t*)digits, n, sh.nw, sh.M, (const uint32_t*)offsets, cursor, sorted);
   pt.mark("sort");
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_accumulate<F>), dim3(blocks_for(tstar, ACC_BLOCK)), dim3(ACC_BLOCK), 0, s,
-                     points, index, n_points, (const uint32_t*)sorted, (const uint32_t*)offsets, nb, tstar,
+                     acc_points, stride, index, n_points, (const uint32_t*)sorted, (const uint32_t*)offsets, nb, tstar,
                      buckets, heads, tails, err);
   pt.mark("accumulate");
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_queue<F>), dim3(blocks_for(nb, QUEUE_BLOCK)), dim3(QUEUE_BLOCK), 0, s,
@@ -954,7 +976,8 @@ static int msm_batched_run(const aff<F>* points, size_t n_points, const uint32_t
     hipLaunchKernelGGL(k_msm_scatter_keys, dim3(blocks_for(E_max, 256)), dim3(256), 0, s, (const uint32_t*)keys,
                        nterms, nw, (const uint32_t*)offsets, cursor, sorted);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_accumulate<F>), dim3(blocks_for(tstar, ACC_BLOCK)), dim3(ACC_BLOCK), 0,
-                       s, points, index, index ? n_points : nterms, (const uint32_t*)sorted, (const uint32_t*)offsets,
+                       s, points, (uint32_t)sizeof(aff<F>), index, index ? n_points : nterms, (const uint32_t*)sorted,
+                       (const uint32_t*)offsets,
                        nb, tstar, buckets, heads, tails, err);
   }
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_msm_fixup_queue<F>), dim3(blocks_for(nb, QUEUE_BLOCK)), dim3(QUEUE_BLOCK), 0, s,

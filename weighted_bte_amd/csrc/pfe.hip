@@ -570,3 +570,103 @@ extern "C" int bte_pfe_partial_decrypt(const bte_p1_affine* ct1, size_t batch_si
   PFE_TRY(g1_msm_batched_gather(ct1, B, d_pidx, fractions, nullptr, N * B, d_off, N, 0, sigma_out, s));
   return BTE_OK;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Plan of the fused weighted_pfe step: bte_pfe_precompute_batched then
+// bte_pfe_open_prepared_batched for `count` batches of one committee (same outputs), with the
+// Cauchy intermediates in plan-owned buffers, and its HIP graph: the first launch captures one
+// run (the entry points allocate nothing eager and own their scratch while captured), later
+// launches replay it.  No counterpart in the reference (its steps are rayon loops).
+// ---------------------------------------------------------------------------------------------
+struct bte_pfe_plan {
+  size_t B = 0, K = 0, T = 0;
+  const bte_fp6 *total_lines = nullptr, *d1_lines = nullptr, *mask_lines = nullptr;
+  std::vector<uint8_t> alpha;
+  bte_p1* first_cauchy = nullptr;
+  bte_fp12* pairing_cauchy = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* bound[4] = {nullptr, nullptr, nullptr, nullptr};
+  ~bte_pfe_plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (first_cauchy) (void)hipFree(first_cauchy);
+    if (pairing_cauchy) (void)hipFree(pairing_cauchy);
+  }
+};
+
+extern "C" int bte_pfe_plan_create(size_t batch_size, size_t count, size_t n_accepted, const bte_fp6* total_lines,
+                                   const bte_fp6* d1_lines, const bte_fp6* mask_lines, const uint8_t* alpha,
+                                   bte_pfe_plan** out, void* stream) {
+  (void)stream;
+  if (!out || !total_lines || !d1_lines || !mask_lines || !alpha)
+    return set_error(BTE_EINVAL, "pfe_plan_create: null pointer");
+  if (batch_size == 0 || (batch_size & (batch_size - 1)) || count == 0 || n_accepted == 0)
+    return set_error(BTE_EINVAL, "pfe_plan_create: bad shape (B must be a power of two)");
+  *out = nullptr;
+  bte_pfe_plan* p = new bte_pfe_plan();
+  p->B = batch_size;
+  p->K = count;
+  p->T = n_accepted;
+  p->total_lines = total_lines;
+  p->d1_lines = d1_lines;
+  p->mask_lines = mask_lines;
+  p->alpha.assign(alpha, alpha + 32 * batch_size);
+  const size_t n = batch_size * count;
+  hipError_t e = hipMalloc((void**)&p->first_cauchy, n * sizeof(bte_p1));
+  if (e == hipSuccess) e = hipMalloc((void**)&p->pairing_cauchy, n * sizeof(bte_fp12));
+  if (e != hipSuccess) {
+    delete p;
+    return set_error(BTE_ENOMEM, "pfe plan: %s", hipGetErrorString(e));
+  }
+  *out = p;
+  return BTE_OK;
+}
+
+static int pfe_plan_enqueue(bte_pfe_plan* p, const bte_p1_affine* ct1, const bte_p1* shares, const bte_fp12* ct2,
+                            bte_fp12* messages_out, void* stream) {
+  PFE_TRY(bte_pfe_precompute_batched(ct1, p->B, p->K, p->total_lines, p->first_cauchy, p->pairing_cauchy, stream));
+  return bte_pfe_open_prepared_batched(shares, p->T, p->d1_lines, p->mask_lines, ct1, ct2, p->first_cauchy,
+                                       p->pairing_cauchy, p->alpha.data(), p->B, p->K, messages_out, stream);
+}
+
+extern "C" int bte_pfe_plan_run(bte_pfe_plan* plan, const bte_p1_affine* ct1, const bte_p1* shares,
+                                const bte_fp12* ct2, bte_fp12* messages_out, void* stream) {
+  if (!plan || !ct1 || !shares || !ct2 || !messages_out) return set_error(BTE_EINVAL, "pfe_plan_run: null pointer");
+  return pfe_plan_enqueue(plan, ct1, shares, ct2, messages_out, stream);
+}
+
+extern "C" int bte_pfe_plan_launch_graph(bte_pfe_plan* plan, const bte_p1_affine* ct1, const bte_p1* shares,
+                                         const bte_fp12* ct2, bte_fp12* messages_out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!plan || !ct1 || !shares || !ct2 || !messages_out) return set_error(BTE_EINVAL, "pfe_plan_graph: null pointer");
+  if (!s) return set_error(BTE_EINVAL, "pfe_plan_graph: the null stream cannot be captured");
+  const void* args[4] = {ct1, shares, ct2, messages_out};
+  if (!plan->exec) {
+    if (PhaseTimer(s).on)  // phase timing synchronises the stream, which a capture cannot contain
+      return set_error(BTE_EINVAL, "pfe_plan_graph: disable profiling before the first launch");
+    BTE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = pfe_plan_enqueue(plan, ct1, shares, ct2, messages_out, stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return set_error(BTE_EDEVICE, "pfe_plan_graph: capture failed: %s", hipGetErrorString(e));
+    plan->graph = g;
+    BTE_HIP(hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0));
+    for (int i = 0; i < 4; ++i) plan->bound[i] = args[i];
+  } else {
+    for (int i = 0; i < 4; ++i)
+      if (plan->bound[i] != args[i])
+        return set_error(BTE_EINVAL, "pfe_plan_graph: the graph is bound to the buffers of its first launch");
+  }
+  BTE_HIP(hipGraphLaunch(plan->exec, s));
+  return BTE_OK;
+}
+
+extern "C" int bte_pfe_plan_destroy(bte_pfe_plan* plan) {
+  delete plan;
+  return BTE_OK;
+}

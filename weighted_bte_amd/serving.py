@@ -1,6 +1,10 @@
-"""Continuous-batching decryption service over the fused weighted_btx entry points.
+"""Continuous-batching decryption service over the fused weighted_btx and weighted_pfe entry
+points.
 
-Requests are batches of weighted_btx ciphertexts with the accepted shares of one committee.
+Requests are batches of weighted_btx or weighted_pfe ciphertexts with the accepted shares of
+one committee.  A weighted_pfe committee has one power-of-two batch size B; its requests are
+completed to B slots (pad_pfe_batch) and all admitted requests of one committee run as one fused
+pfe precompute + open.
 The server keeps each committee's key material and prepared state (kernel line tables, opening
 keys and the opening keys' line tables, per padded batch size) resident in HBM.  Whenever requests are waiting and HBM
 budget and a launch slot are free, it admits the waiting requests that fit and runs all of them,
@@ -25,6 +29,7 @@ import torch
 from . import api
 
 G1A, G1J, F12 = 96, 144, 576
+_P = 0x1A0111EA397FE69A4B1BA7B6434BACD764774B84F38512BF6730D2A0F6B0F6241EABFFFEB153FFFFB9FEFFFFFFFFAAAB
 
 
 def next_pow2(b):
@@ -74,6 +79,47 @@ class Committee:
         return self.prepared[bp]
 
 
+@dataclass
+class PfeCommittee:
+    """A resident weighted_pfe committee: key material (d1 then d2), its power-of-two batch size B
+    (weighted_pfe/src/decryption.rs:115-123: every batch has exactly B slots), alpha points,
+    global key, and the prepared state (d1 by party and its line tables, total and mask lines)."""
+    material: torch.Tensor
+    total_weight: int
+    batch_size: int
+    offsets: list
+    accepted: list
+    alpha: list
+    global_key: torch.Tensor
+    state: tuple = None
+
+    @property
+    def max_batch(self):
+        return self.batch_size
+
+    def prepare(self, bp=None):
+        if self.state is None:
+            d1p, tl, ml = api.pfe_prepare(self.material, self.total_weight, self.batch_size, self.offsets,
+                                          self.accepted, self.alpha, self.global_key)
+            self.state = (d1p, tl, ml, api.prepare_g2_lines(d1p))
+        return self.state
+
+
+def pad_pfe_batch(ct1, ct2, batch_size):
+    """A weighted_pfe request of b < B ciphertexts completed to the committee's B slots: the empty
+    slots hold the identity in G1 and one in GT, so they add nothing to any party's share or to
+    the Cauchy transforms of the real slots (the parties compute their shares over this padded
+    batch); their messages are discarded."""
+    b = ct1.numel() // G1A
+    if b == batch_size:
+        return ct1, ct2
+    one = torch.zeros(F12, dtype=torch.uint8, device=ct2.device)
+    one[:48] = torch.tensor(list(((1 << 384) % _P).to_bytes(48, "little")), dtype=torch.uint8, device=ct2.device)
+    pad1 = torch.zeros((batch_size - b) * G1A, dtype=torch.uint8, device=ct1.device)
+    pad2 = one.repeat(batch_size - b)
+    return torch.cat([ct1, pad1]), torch.cat([ct2, pad2])
+
+
 class BtxServer:
     def __init__(self, committees, hbm_budget=8 << 30, max_fuse=256, max_inflight=2):
         self.committees = committees
@@ -90,10 +136,16 @@ class BtxServer:
         """Queue a request.  Any 1 <= B <= L is accepted (validate_batch_size,
         weighted_btx/src/decryption.rs:711-722); it runs against the state prepared for
         min(nextpow2(B), L), whose FFT size nextpow2(2 B') is that of nextpow2(B)."""
-        L = self.committees[req.committee].max_batch
+        com = self.committees[req.committee]
+        L = com.max_batch
         if req.batch < 1 or req.batch > L:
             raise ValueError("BatchTooLarge")
-        req.padded = min(next_pow2(req.batch), L)
+        if isinstance(com, PfeCommittee):
+            if req.ct1.numel() != com.batch_size * G1A:
+                raise ValueError("a weighted_pfe request carries the committee's B slots (pad_pfe_batch)")
+            req.padded = com.batch_size
+        else:
+            req.padded = min(next_pow2(req.batch), L)
         self.waiting.append(req)
 
     def _used(self):
@@ -114,21 +166,39 @@ class BtxServer:
         return admitted
 
     def _launch(self, reqs, stream):
-        work = []
+        work, btx_reqs, pfe_groups = [], [], {}
         for r in reqs:
             com = self.committees[r.committee]
+            if isinstance(com, PfeCommittee):
+                pfe_groups.setdefault(r.committee, []).append(r)
+                continue
             bp = r.padded
             lines, keys, key_lines = com.prepare(bp)
+            btx_reqs.append(r)
             work.append({"ct1": r.ct1, "ct2": r.ct2, "valid": r.valid or [True] * r.batch, "batch_size": bp,
                          "lines": lines, "keys": keys, "key_lines": key_lines, "shares": r.shares,
                          "n_accepted": len(com.accepted)})
+        for cid in pfe_groups:
+            self.committees[cid].prepare()
         stream.wait_stream(torch.cuda.current_stream())  # inputs and prepared state
         with torch.cuda.stream(stream):
-            outs = api.btx_decrypt_requests(work)
+            if work:
+                for r, o in zip(btx_reqs, api.btx_decrypt_requests(work)):
+                    r.messages = o
+            # weighted_pfe: every admitted request of one committee in one fused precompute + open
+            for cid, group in pfe_groups.items():
+                com = self.committees[cid]
+                d1p, tl, ml, d1l = com.state
+                B, T = com.batch_size, len(com.accepted)
+                ct1 = torch.cat([r.ct1 for r in group])
+                ct2 = torch.cat([r.ct2 for r in group])
+                shares = torch.cat([r.shares for r in group])
+                fc, pc = api.pfe_precompute_batched(ct1, tl, B)
+                out = api.pfe_open_prepared_batched(shares, T, d1l, ml, ct1, ct2, fc, pc, com.alpha, B)
+                for k, r in enumerate(group):
+                    r.messages = out[k * B * F12:(k * B + r.batch) * F12]
             ev = torch.cuda.Event()
             ev.record(stream)
-        for r, o in zip(reqs, outs):
-            r.messages = o
         return ev
 
     def poll(self, now_fn=time.perf_counter):
@@ -174,8 +244,14 @@ class BtxServer:
         return done
 
     def decrypt_alone(self, req: Request):
-        """The same request through the plain per-call path (no fusion, no padding)."""
+        """The same request through the plain per-call path (no fusion; no padding for btx, the
+        committee's fixed B for pfe)."""
         com = self.committees[req.committee]
+        if isinstance(com, PfeCommittee):
+            d1p, tl, ml, _ = com.prepare()
+            fc, pc = api.pfe_precompute(req.ct1, tl)
+            out = api.pfe_open(req.shares, d1p, ml, req.ct1, req.ct2, fc, pc, com.alpha)
+            return out[:req.batch * F12]
         lines, keys = api.btx_prepare(com.material, com.total_weight, com.max_batch, com.offsets,
                                       com.accepted, req.batch)
         valid = req.valid or [True] * req.batch
