@@ -215,14 +215,10 @@ def _affine_to_jac(api, aff, group):
     return torch.cat([aff.view(n, w), z], dim=1).reshape(-1).contiguous()
 
 
-def bench_multi_pairing(nslots=512, seed=11):
-    """Per-ciphertext multi-pairings of uneven arity (log-uniform in 2..512 pairs) in one fused
-    launch sequence with one shared batched final exponentiation: unprepared (G2 points) and
-    prepared (G2 line tables); parity against gT^(sum a_i b_i) for pairs (a_i G1, b_i G2)."""
+def _multi_pairing_inputs(nslots=512, seed=11):
+    """Pairs (a_i G1, b_i G2) in nslots segments of arity log-uniform in 2..512."""
     import math
     import random
-
-    import torch
 
     from weighted_bte_amd import api, inputs
 
@@ -236,6 +232,50 @@ def bench_multi_pairing(nslots=512, seed=11):
     a_s, b_s = st.scalars(npairs), st.scalars(npairs)
     P = api.g1_generator_mul(api.to_device(inputs.pack_scalars(a_s)))
     Q = api.g2_generator_mul(api.to_device(inputs.pack_scalars(b_s)))
+    return P, Q, off, a_s, b_s
+
+
+def bench_knob_tuning(bases, scalars):
+    """Autotunes the launch-shape knobs (bte_tune_set) on their own workloads and reports
+    default against tuned: the bucket-accumulation rounds on the headline MSM, the three-lane
+    switch of the cyclotomic stages (batched prefix inverse transforms of size 1024, 2..128 at
+    a time) and the pairs per multi-Miller accumulator (512 prepared multi-pairings of arity
+    2..512).  Runs after every other measurement and leaves the built-in values installed."""
+    from weighted_bte_amd import api, autotune
+
+    out = {}
+    dr = api.TUNE_DEFAULTS[api.TUNE_MSM_ACC_ROUNDS]
+    best, times = autotune.tune_acc_rounds(bases, scalars)
+    out["msm_acc_rounds"] = {"default": dr, "default_ms": times.get(dr), "tuned": best, "tuned_ms": times[best],
+                             "sweep_ms": times, "what": "full-occupancy rounds of accumulation threads, G1 MSM 2^20"}
+    print("tuned accumulation rounds: %d (default %d)" % (best, dr), file=sys.stderr, flush=True)
+    d3 = api.TUNE_DEFAULTS[api.TUNE_CYC_THREE_LANE_MAX]
+    thr, rows = autotune.tune_three_lane(1024)
+    out["cyc_three_lane_max"] = {"default": d3, "tuned": thr, "table_ms": rows,
+                                 "what": "stages narrower than this run each Fp12 on three lanes"}
+    print("tuned three-lane switch: %d (default %d)" % (thr, d3), file=sys.stderr, flush=True)
+    P, Q, off, _, _ = _multi_pairing_inputs()
+    lines = api.prepare_g2_lines(Q)
+    dm = api.TUNE_DEFAULTS[api.TUNE_MM_CHUNK_MAX]
+    best, times = autotune.tune_mm_chunk(lines, P, off)
+    out["mm_chunk_max"] = {"default": dm, "default_ms": times.get(dm), "tuned": best, "tuned_ms": times[best],
+                           "sweep_ms": times, "what": "pairs sharing one Miller accumulator, 512 uneven multi-pairings"}
+    print("tuned multi-Miller chunk: %d (default %d)" % (best, dm), file=sys.stderr, flush=True)
+    for knob, v in api.TUNE_DEFAULTS.items():
+        api.tune_set(knob, v)
+    return out
+
+
+def bench_multi_pairing(nslots=512, seed=11):
+    """Per-ciphertext multi-pairings of uneven arity (log-uniform in 2..512 pairs) in one fused
+    launch sequence with one shared batched final exponentiation: unprepared (G2 points) and
+    prepared (G2 line tables); parity against gT^(sum a_i b_i) for pairs (a_i G1, b_i G2)."""
+    import torch
+
+    from weighted_bte_amd import api
+
+    P, Q, off, a_s, b_s = _multi_pairing_inputs(nslots, seed)
+    npairs = off[-1]
     un, ms_un = _timed(lambda: api.full_final_exponentiation(api.unprepared_multi_miller_loop(P, Q, off)))
     lines, ms_lines = _timed(lambda: api.prepare_g2_lines(Q), reps=1)
     pr, ms_pr = _timed(lambda: api.full_final_exponentiation(api.multi_miller_loop_prepared(lines, P, off)))
@@ -682,6 +722,14 @@ def main():
                                    weight_per_party=16) if a.serving_requests > 0 and not a.no_serving_large
                      else None)
 
+    knobs = None
+    if not a.no_tune:  # last, so nothing measured above runs after a sweep of these knobs
+        try:
+            knobs = bench_knob_tuning(bases, dev_sc)
+        except Exception as e:  # report, and keep the line
+            knobs = {"error": str(e)[:300]}
+            print("knob tuning failed: %s" % e, file=sys.stderr, flush=True)
+
     line = {
         "metric": "G1 MSM 2^%d ms (decrypted ctxts/s at B=512 reported separately)" % a.logn,
         "value": round(ms, 4),
@@ -698,6 +746,7 @@ def main():
         "config": {"workload": "g1_msm", "n": n, "window_bits": c, "sparse_fraction": 0.1,
                    "parallelism": "single GPU"},
         "autotuned": tuned,
+        "autotuned_knobs": knobs,
         "parity": "closed-form ok" if parity else "MISMATCH",
         "msm_per_s": round(1e3 / ms, 3),
         "phases_ms": {k: round(v, 4) for k, v in phases.items()},

@@ -446,6 +446,25 @@ int bte_profile_read(char* buf, size_t len);
 /* Measured 32x32->64 multiply-accumulates per second of the whole device. */
 int bte_probe_mac_rate(double* macs_per_second, void* stream);
 
+/* Launch-shape knobs (no counterpart in the reference: blst and rayon pick their own work
+ * split; results never depend on these).  Set by the autotuner (weighted_bte_amd/autotune.py),
+ * read by a driver when it sizes a launch; a plan keeps the values it was created with.
+ *   BTE_TUNE_MSM_ACC_ROUNDS      full-occupancy rounds of bucket-accumulation threads, 1..16
+ *                                (default 2): fewer rounds mean longer chunks per thread
+ *   BTE_TUNE_CYC_THREE_LANE_MAX  cyclotomic stages with fewer butterflies than this run each
+ *                                Fp12 on three lanes, 0..2^30 (default 32768)
+ *   BTE_TUNE_MM_CHUNK_MAX        pairs sharing one multi-Miller accumulator, 1..64 (default 8)
+ *   BTE_TUNE_MM_MIN_THREADS      fewest multi-Miller threads before chunks shrink, 1..2^24
+ *                                (default 65536) */
+enum {
+  BTE_TUNE_MSM_ACC_ROUNDS = 1,
+  BTE_TUNE_CYC_THREE_LANE_MAX = 2,
+  BTE_TUNE_MM_CHUNK_MAX = 3,
+  BTE_TUNE_MM_MIN_THREADS = 4
+};
+int bte_tune_set(int knob, int64_t value);
+int bte_tune_get(int knob, int64_t* value);
+
 /* Test support (no counterpart in the reference): the nth allocation from now of the library's
  * cached tables (FFT domain twiddles and plans) fails as if the device were out of memory (0
  * cancels), so a test can check that a failed build leaves nothing half-made behind. */

@@ -1,0 +1,85 @@
+"""The launch-shape knobs the autotuner sets (bte_tune_set) change how work is split across
+threads, never a result: MSMs with other accumulation chunk lengths give the same points, the
+cyclotomic prefix transform on one lane or three lanes per element gives the same raw values,
+and multi-Miller loops with other accumulator chunks give the same raw Fp12 (the shared
+squarings distribute over the product, final_exponentiation.rs:229-273)."""
+import ctypes
+import random
+
+import pytest
+
+from helpers import O, g1_mul_gen, g2_mul_gen
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api(dev):
+    from weighted_bte_amd import api as A
+    yield A
+    for knob, default in A.TUNE_DEFAULTS.items():
+        A.tune_set(knob, default)
+
+
+def test_msm_same_points_for_every_accumulation_round_count(api):
+    from weighted_bte_amd import inputs
+
+    n = 1 << 14
+    st = inputs.Stream(31, 14)
+    ks, sc = st.scalars(n), inputs.msm_scalars(st, n)
+    bases = api.g1_generator_mul(api.to_device(inputs.pack_scalars(ks)))
+    dsc = api.to_device(inputs.pack_scalars(sc))
+    want = api.to_host(api.g1_generator_mul(api.to_device((sum(k * s for k, s in zip(ks, sc)) % O.R).to_bytes(32, "little"))))
+    for rounds in (1, 2, 4, 16):
+        api.tune_set(api.TUNE_MSM_ACC_ROUNDS, rounds)
+        got = api.to_host(api.batch_normalize_g1(api.g1_multi_exp_affine_bytes(bases, dsc)))
+        assert got == want, rounds
+    api.tune_set(api.TUNE_MSM_ACC_ROUNDS, 2)
+
+
+def test_cyclotomic_prefix_same_raw_values_on_one_or_three_lanes(api):
+    import torch
+
+    from weighted_bte_amd import call, workloads
+
+    n, count, outs = 64, 12, 32
+    rnd = random.Random(17)
+    gt = workloads.gt_generator_bytes()
+    x = api.cyclotomic_mul_scalar_decomposed(api.to_device(gt * (n * count)),
+                                             [rnd.randrange(O.R) for _ in range(n * count)])
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    results = []
+    for thr in (0, 1 << 30, 100):  # every stage one lane, every stage three lanes, a split
+        api.tune_set(api.TUNE_CYC_THREE_LANE_MAX, thr)
+        a = x.clone()
+        call("bte_ifft_cyclotomic_prefix_unscaled_batched", ctypes.c_void_p(a.data_ptr()), n, count, outs, stream)
+        results.append(a.view(count, n * 576)[:, :outs * 576].clone())
+    api.tune_set(api.TUNE_CYC_THREE_LANE_MAX, api.TUNE_DEFAULTS[api.TUNE_CYC_THREE_LANE_MAX])
+    assert torch.equal(results[0], results[1]) and torch.equal(results[0], results[2])
+    # and the values are the oracle's prefix transform of the first element
+    first = [O.f12_from_bytes(api.to_host(x[576 * i:576 * (i + 1)])) for i in range(n)]
+    want = O.ifft_cyclotomic_prefix_unscaled(first, outs)[:outs]
+    got = api.to_host(results[0][0])
+    assert [O.f12_from_bytes(got[576 * i:576 * (i + 1)]) for i in range(outs)] == want
+
+
+def test_multi_miller_same_raw_values_for_every_chunk(api):
+    import torch
+
+    rnd = random.Random(23)
+    arity = [1, 5, 0, 9, 2, 17]
+    off = [0]
+    for a in arity:
+        off.append(off[-1] + a)
+    npairs = off[-1]
+    P = api.to_device(b"".join(O.g1_affine_to_bytes(g1_mul_gen(rnd.randrange(1, O.R))) for _ in range(npairs)))
+    Q = api.to_device(b"".join(O.g2_affine_to_bytes(g2_mul_gen(rnd.randrange(1, O.R))) for _ in range(npairs)))
+    lines = api.prepare_g2_lines(Q)
+    outs = []
+    for chunk in (1, 3, 8, 64):
+        api.tune_set(api.TUNE_MM_CHUNK_MAX, chunk)
+        api.tune_set(api.TUNE_MM_MIN_THREADS, 1)  # let the chunk reach its maximum at this size
+        outs.append(api.multi_miller_loop_prepared(lines, P, off).clone())
+    for knob in (api.TUNE_MM_CHUNK_MAX, api.TUNE_MM_MIN_THREADS):
+        api.tune_set(knob, api.TUNE_DEFAULTS[knob])
+    assert all(torch.equal(outs[0], o) for o in outs[1:])

@@ -85,3 +85,29 @@ def test_cyclotomic_prefix_work_counter_matches_oracle_plans():
                 pow_work(pow(winv, k, O.R))
             mul += 1
         assert list(counts) == [sqr, mul, frob], n
+
+
+def test_tune_knobs_round_trip_and_range_checks():
+    """bte_tune_set / bte_tune_get (launch-shape knobs, host-only): defaults, a round trip, and
+    out-of-range values or unknown knobs refused with BTE_EINVAL and the value left unchanged."""
+    import weighted_bte_amd
+    from weighted_bte_amd import api
+
+    lib = weighted_bte_amd.lib()
+    for knob, default in api.TUNE_DEFAULTS.items():
+        assert api.tune_get(knob) == default
+    try:
+        api.tune_set(api.TUNE_MSM_ACC_ROUNDS, 3)
+        assert api.tune_get(api.TUNE_MSM_ACC_ROUNDS) == 3
+        api.tune_set(api.TUNE_CYC_THREE_LANE_MAX, 0)
+        assert api.tune_get(api.TUNE_CYC_THREE_LANE_MAX) == 0
+        for knob, bad in ((api.TUNE_MSM_ACC_ROUNDS, 0), (api.TUNE_MSM_ACC_ROUNDS, 17), (api.TUNE_MM_CHUNK_MAX, 65),
+                          (api.TUNE_MM_MIN_THREADS, 0), (api.TUNE_CYC_THREE_LANE_MAX, -1)):
+            before = api.tune_get(knob)
+            assert lib.bte_tune_set(knob, bad) == 1
+            assert api.tune_get(knob) == before
+        assert lib.bte_tune_set(9, 1) == 1
+        assert b"unknown knob" in lib.bte_last_error()
+    finally:
+        for knob, default in api.TUNE_DEFAULTS.items():
+            api.tune_set(knob, default)

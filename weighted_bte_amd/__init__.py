@@ -40,6 +40,8 @@ _SIGNATURES = {
     "bte_g2_generator_mul": [_vp, _sz, _vp, _vp],
     "bte_fft_g1": [_vp, _sz, _i, _i, _vp],
     "bte_fft_set_dft_max_log": [_i],
+    "bte_tune_set": [_i, ctypes.c_int64],
+    "bte_tune_get": [_i, _vp],
     "bte_fft_g2": [_vp, _sz, _i, _i, _vp],
     "bte_fft_gt": [_vp, _sz, _i, _i, _vp],
     "bte_fft_cyclotomic": [_vp, _sz, _i, _vp],

@@ -197,6 +197,22 @@ def set_msm_window(group, n, bits):
     call("bte_msm_set_window", group, n, bits)
 
 
+# Launch-shape knobs of bte_tune_set (include/bte_hip.h); results never depend on them.
+TUNE_MSM_ACC_ROUNDS, TUNE_CYC_THREE_LANE_MAX, TUNE_MM_CHUNK_MAX, TUNE_MM_MIN_THREADS = 1, 2, 3, 4
+TUNE_DEFAULTS = {TUNE_MSM_ACC_ROUNDS: 2, TUNE_CYC_THREE_LANE_MAX: 32768, TUNE_MM_CHUNK_MAX: 8,
+                 TUNE_MM_MIN_THREADS: 65536}
+
+
+def tune_set(knob, value):
+    call("bte_tune_set", knob, int(value))
+
+
+def tune_get(knob):
+    v = ctypes.c_int64()
+    call("bte_tune_get", knob, ctypes.byref(v))
+    return v.value
+
+
 # --- a4..a9: Radix2Domain transforms (weighted_btx/src/fft.rs, weighted_pfe/src/fft.rs) -----
 class Radix2Domain:
     """Radix2Domain::new (fft.rs:164-191): the size must be a nonzero power of two <= 2^32."""

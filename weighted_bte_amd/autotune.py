@@ -1,13 +1,24 @@
-"""Autotuner for the Pippenger window width (the plain per-call MSM picks
-c = clamp(log2 n - 4, 4, 16); blst picks its own, blst_utils.rs:57-64, and the result does not
-depend on it).
+"""Autotuner for the launch shapes of the hot path.  None of them changes a result; blst and
+rayon make the corresponding choices themselves in the reference (Pippenger window,
+blst_utils.rs:57-64; chunk and pair scheduling of the transforms, fft.rs:370-386).
 
-    best, times = tune_msm(1, points, scalars)      # sweeps c around the default, sets the best
-    save_table({("g1", 20): best})                  # keyed by device name
+  * Pippenger window width c per (group, log2 n): the plain per-call MSM picks
+    c = clamp(log2 n - 4, 4, 16).
+        best, times = tune_msm(1, points, scalars)
+  * bucket-accumulation rounds (BTE_TUNE_MSM_ACC_ROUNDS): chunk length of the MSM's sorted
+    entries per thread, as full-occupancy rounds of the device.
+        best, times = tune_acc_rounds(points, scalars)
+  * three-lane switch of the cyclotomic stages (BTE_TUNE_CYC_THREE_LANE_MAX): the stage width
+    below which each Fp12 runs on three lanes instead of one.
+        best, table = tune_three_lane(n=1024)
+  * multi-Miller chunk (BTE_TUNE_MM_CHUNK_MAX): pairs sharing one Miller accumulator.
+        best, times = tune_mm_chunk(lines, points, seg_offsets)
+
+    save_table({("g1", 20): best}, knobs={...})     # keyed by device name
     apply_table()                                   # re-applies saved choices in a new process
 
 Times are wall-clock per call on inputs already in HBM (best of `reps` after one warm-up), so
-they include the host-side window combination the per-call entry point performs.
+they include the host-side work the per-call entry points perform.
 """
 from __future__ import annotations
 
@@ -64,12 +75,94 @@ def tune_msm(group, points, scalars, candidates=None, reps=3):
     return best, times
 
 
+def _time(fn, reps=3):
+    fn()
+    torch.cuda.synchronize()
+    best = math.inf
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t)
+    return round(best * 1e3, 4)
+
+
+def tune_knob(knob, candidates, workload, reps=3):
+    """Times workload() under each candidate value of a bte_tune_set knob, installs the fastest
+    and returns (best, {value: ms})."""
+    old = api.tune_get(knob)
+    times = {}
+    try:
+        for v in candidates:
+            api.tune_set(knob, v)
+            times[v] = _time(workload, reps)
+    finally:
+        api.tune_set(knob, old)
+    best = min(times, key=times.get)
+    api.tune_set(knob, best)
+    return best, times
+
+
+def tune_acc_rounds(points, scalars, group=1, candidates=(1, 2, 3, 4)):
+    """Bucket-accumulation rounds for one MSM shape (the window already installed)."""
+    fn = _msm_fn(group)
+    return tune_knob(api.TUNE_MSM_ACC_ROUNDS, candidates, lambda: fn(points, scalars))
+
+
+def tune_mm_chunk(lines, points, seg_offsets, candidates=(2, 4, 8, 16)):
+    """Pairs per multi-Miller accumulator on a prepared multi-pairing workload."""
+    return tune_knob(api.TUNE_MM_CHUNK_MAX, candidates,
+                     lambda: api.multi_miller_loop_prepared(lines, points, seg_offsets))
+
+
+def tune_three_lane(n=1024, counts=(2, 8, 16, 32, 64, 128), reps=2):
+    """Times the batched prefix inverse cyclotomic transform of `count` size-n transforms with
+    every stage on one lane per Fp12 and with every stage on three lanes, and installs the
+    switch at the narrowest stage width where one lane wins.  Returns (threshold, rows)."""
+    from . import call
+    from .api import _ptr, _stream
+
+    rows = []
+    old = api.tune_get(api.TUNE_CYC_THREE_LANE_MAX)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    try:
+        for count in counts:
+            # any Fp12 values time the same: below p limb-wise, top limb masked
+            v = torch.randint(0, 1 << 31, (count * n * 144,), dtype=torch.int64, generator=g)
+            v = v.to(torch.int32).view(count * n, 12, 12)
+            v[:, :, 11] &= 0x0FFFFFFF
+            base = v.reshape(-1).to("cuda")
+            row = {"count": count, "stage_butterflies": n // 2 * count}
+            for name, thr in (("one_lane_ms", 0), ("three_lane_ms", 1 << 30)):
+                api.tune_set(api.TUNE_CYC_THREE_LANE_MAX, thr)
+                work = base.clone()
+                row[name] = _time(lambda: call("bte_ifft_cyclotomic_prefix_unscaled_batched", _ptr(work), n, count,
+                                               n // 2, _stream()), reps)
+            rows.append(row)
+    finally:
+        api.tune_set(api.TUNE_CYC_THREE_LANE_MAX, old)
+    # the narrowest measured width from which one lane wins at every wider width; if three
+    # lanes still win at the widest width measured, switch just above it
+    rows.sort(key=lambda r: r["stage_butterflies"])
+    thr = 2 * rows[-1]["stage_butterflies"] if rows else old
+    for r in reversed(rows):
+        if r["one_lane_ms"] > r["three_lane_ms"]:
+            break
+        thr = r["stage_butterflies"]
+    api.tune_set(api.TUNE_CYC_THREE_LANE_MAX, thr)
+    return thr, rows
+
+
+KNOBS = {"msm_acc_rounds": 1, "cyc_three_lane_max": 2, "mm_chunk_max": 3, "mm_min_threads": 4}
+
+
 def _device_key():
     return torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
 
 
-def save_table(choices, path=TABLE):
-    """choices: {("g1" | "g2", log2 n): c}; merged into the table of this device."""
+def save_table(choices, path=TABLE, knobs=None):
+    """choices: {("g1" | "g2", log2 n): c}; knobs: {name in KNOBS: value}; merged into the
+    table of this device."""
     data = {}
     if os.path.exists(path):
         with open(path) as f:
@@ -77,6 +170,10 @@ def save_table(choices, path=TABLE):
     dev = data.setdefault(_device_key(), {})
     for (group, logn), c in choices.items():
         dev["%s_%d" % (group, logn)] = int(c)
+    for name, v in (knobs or {}).items():
+        if name not in KNOBS:
+            raise ValueError("unknown knob %r" % name)
+        dev[name] = int(v)
     os.makedirs(os.path.dirname(path), exist_ok=True)
     with open(path, "w") as f:
         json.dump(data, f, indent=1, sort_keys=True)
@@ -90,7 +187,10 @@ def apply_table(path=TABLE):
         data = json.load(f)
     applied = 0
     for key, c in data.get(_device_key(), {}).items():
-        group, logn = key.split("_")
-        api.set_msm_window(1 if group == "g1" else 2, 1 << int(logn), int(c))
+        if key in KNOBS:
+            api.tune_set(KNOBS[key], int(c))
+        else:
+            group, logn = key.split("_")
+            api.set_msm_window(1 if group == "g1" else 2, 1 << int(logn), int(c))
         applied += 1
     return applied

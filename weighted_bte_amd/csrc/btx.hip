@@ -668,6 +668,7 @@ struct bte_btx_plan {
   const bte_fp6** tab = nullptr;
   uint8_t* valid = nullptr;
   uint32_t* cyctab = nullptr;
+  size_t c3max = 0;  // three-lane switch the plan was created with
   uint64_t *lo = nullptr, *hi = nullptr, *seg = nullptr;
   std::vector<void*> allocs;
   hipGraph_t graph = nullptr;
@@ -709,7 +710,8 @@ extern "C" int bte_btx_plan_create(size_t batch_size, size_t count, size_t n_acc
   MMChunks ch;
   mm_chunks(lo.data(), hi.data(), S, ch);
   p->nchunks = ch.lo.size();
-  const size_t ctab = cyc_tab_bytes(p->m, p->K);
+  p->c3max = (size_t)tune_get(BTE_TUNE_CYC_THREE_LANE_MAX);
+  const size_t ctab = cyc_tab_bytes(p->m, p->K, p->c3max);
   int rc = BTE_OK;
   do {
     if ((rc = p->alloc(&p->c, M)) || (rc = p->alloc(&p->ca, M)) || (rc = p->alloc(&p->ml, M)) ||
@@ -747,7 +749,7 @@ static int plan_enqueue(bte_btx_plan* p, const bte_p1_affine* ct1, const bte_p1*
   BTE_TRY(bte_g1_batch_normalize((const bte_p1*)p->c, M, (bte_p1_affine*)p->ca, s));
   BTE_TRY(bte_miller_loop_prepared_tiled(p->lines, m, (const bte_p1_affine*)p->ca, M, (bte_fp12*)p->ml, s));
   BTE_TRY(bte_final_exp_easy((const bte_fp12*)p->ml, M, (bte_fp12*)p->conv, s));
-  BTE_TRY(cyc_prefix_batched_enqueue(p->conv, m, K, B, p->cyctab, s));
+  BTE_TRY(cyc_prefix_batched_enqueue(p->conv, m, K, B, p->cyctab, p->c3max, s));
   hipLaunchKernelGGL(k_gather_prefix, dim3(blocks_for(S, 256)), dim3(256), 0, s, (const fp12*)p->conv, m, B, K,
                      p->pref);
   BTE_TRY(bte_final_exp_hard((const bte_fp12*)p->pref, S, (bte_fp12*)p->beta, s));

@@ -21,6 +21,19 @@ int set_error(int code, const char* fmt, ...) {
   return code;
 }
 
+// Launch-shape knobs (bte_tune_set): index = knob id, with their allowed ranges.
+struct TuneKnob {
+  std::atomic<int64_t> value;
+  int64_t lo, hi;
+};
+static TuneKnob g_tune[5] = {{{0}, 0, 0},
+                             {{2}, 1, 16},
+                             {{32768}, 0, (int64_t)1 << 30},
+                             {{8}, 1, 64},
+                             {{65536}, 1, (int64_t)1 << 24}};
+
+int64_t tune_get(int knob) { return g_tune[knob].value.load(std::memory_order_relaxed); }
+
 // Test support: the g_fail_at-th allocation from now made by the library's table caches fails as
 // if the device were out of memory (bte_test_fail_alloc); 0 means none.
 static std::atomic<int> g_fail_at{0};
@@ -138,6 +151,23 @@ extern "C" const char* bte_last_error(void) { return g_last_error; }
 extern "C" int bte_test_fail_alloc(int nth) {
   if (nth < 0) return set_error(BTE_EINVAL, "test_fail_alloc: negative count");
   g_fail_at.store(nth);
+  return BTE_OK;
+}
+
+extern "C" int bte_tune_set(int knob, int64_t value) {
+  if (knob < BTE_TUNE_MSM_ACC_ROUNDS || knob > BTE_TUNE_MM_MIN_THREADS)
+    return set_error(BTE_EINVAL, "tune_set: unknown knob %d", knob);
+  if (value < g_tune[knob].lo || value > g_tune[knob].hi)
+    return set_error(BTE_EINVAL, "tune_set: knob %d value %lld outside %lld..%lld", knob, (long long)value,
+                     (long long)g_tune[knob].lo, (long long)g_tune[knob].hi);
+  g_tune[knob].value.store(value);
+  return BTE_OK;
+}
+
+extern "C" int bte_tune_get(int knob, int64_t* value) {
+  if (knob < BTE_TUNE_MSM_ACC_ROUNDS || knob > BTE_TUNE_MM_MIN_THREADS || !value)
+    return set_error(BTE_EINVAL, "tune_get: bad arguments");
+  *value = tune_get(knob);
   return BTE_OK;
 }
 

@@ -15,6 +15,10 @@ namespace bte {
 // Record the first failure for bte_last_error() and return its status code.
 int set_error(int code, const char* fmt, ...);
 
+// Launch-shape knobs set through bte_tune_set (capi.hip); drivers read them when they size a
+// launch, and plans keep the values they were created with.
+int64_t tune_get(int knob);
+
 // True when a failure was injected for the next cache allocation (bte_test_fail_next_allocs).
 bool test_alloc_should_fail();
 
@@ -104,8 +108,9 @@ int cyc_pow_small_tiled(fp12* v, const int64_t* coefficients, size_t B, size_t K
 // Launch-only pieces for plans and graph capture (fft.hip): odd-power storage of the three-lane
 // cyclotomic stages, the prefix inverse transform over `count` transforms with caller storage,
 // and domain tables built ahead of time.
-size_t cyc_tab_bytes(size_t n, size_t count);
-int cyc_prefix_batched_enqueue(fp12* v, size_t n, size_t count, size_t out_count, uint32_t* tab, hipStream_t s);
+size_t cyc_tab_bytes(size_t n, size_t count, size_t three_lane_max);
+int cyc_prefix_batched_enqueue(fp12* v, size_t n, size_t count, size_t out_count, uint32_t* tab, size_t three_lane_max,
+                               hipStream_t s);
 int fft_warm(size_t n, bool inverse, bool plans, hipStream_t s);
 
 // Segment holding flat index g: off[f] <= g < off[f + 1], off[0] = 0, count segments.

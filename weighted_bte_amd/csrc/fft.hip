@@ -333,9 +333,8 @@ __global__ void __launch_bounds__(64) k_group_scale(jac<F>* __restrict__ v, size
 // element runs on three lanes (coop12.hpp, *3 kernels: 21 butterflies per 64-lane workgroup,
 // `tab` holding the per-lane exponentiation table, C3_TAB_BYTES per lane of the
 // grid), which shortens the per-stage latency of small transforms.  A stage with fewer than
-// C3_MAX_BUTTERFLIES butterflies takes the three-lane form.
+// BTE_TUNE_CYC_THREE_LANE_MAX butterflies (default 32768) takes the three-lane form.
 static constexpr int C3_PER_WG = 21;
-static constexpr size_t C3_MAX_BUTTERFLIES = 32768;
 
 BTE_DEV c3tab c3_tab_for(uint32_t* tab) {
   return {tab, (size_t)gridDim.x * 64, blockIdx.x * (size_t)64 + (threadIdx.x & 63)};
@@ -552,15 +551,18 @@ static int group_fft(jac<F>* v, size_t n, size_t count, bool inverse, bool scale
 
 // Bytes of per-lane odd-power storage the cyclotomic stages of `count` transforms of size n
 // need (zero when they run one thread per butterfly).
-size_t cyc_tab_bytes(size_t n, size_t count) {
+size_t cyc_tab_bytes(size_t n, size_t count, size_t three_lane_max) {
   const size_t nbf = n / 2 * count;
-  return nbf && nbf < C3_MAX_BUTTERFLIES ? (size_t)c3_blocks(nbf) * 64 * C3_TAB_BYTES : 0;
+  return nbf && nbf < three_lane_max ? (size_t)c3_blocks(nbf) * 64 * C3_TAB_BYTES : 0;
 }
 
+static size_t three_lane_max() { return (size_t)tune_get(BTE_TUNE_CYC_THREE_LANE_MAX); }
+
 // mode 0: full transform; mode 1: prefix inverse (last stage only for `out_count` outputs).
-// ext_tab: caller-owned storage of cyc_tab_bytes(n, count) bytes, or null to allocate here.
+// ext_tab: caller-owned storage of cyc_tab_bytes(n, count, c3max) bytes, or null to allocate here;
+// stages with fewer than c3max butterflies take the three-lane form.
 static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool scaled, size_t out_count, int mode,
-                          hipStream_t s, uint32_t* ext_tab = nullptr) {
+                          hipStream_t s, uint32_t* ext_tab = nullptr, size_t c3max = three_lane_max()) {
   const int logn = log2_exact(n);
   if (logn < 0) return set_error(BTE_EINVAL, "fft: InvalidBatchSize(%zu)", n);
   if (count == 0) return BTE_OK;
@@ -571,7 +573,7 @@ static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool sc
   int rc = domain_tables(logn, inverse, true, &t, s);
   if (rc) return rc;
   const size_t nbf = n / 2 * count;
-  const bool three = nbf < C3_MAX_BUTTERFLIES;
+  const bool three = nbf < c3max;
   CallScope sc(s);
   uint32_t* tab = ext_tab;  // exponentiation tables, C3_TAB_BYTES per lane
   if (three && !tab) BTE_TRY_RC(sc.dev((uint8_t**)&tab, (size_t)c3_blocks(nbf) * 64 * C3_TAB_BYTES));
@@ -603,9 +605,9 @@ static int cyclotomic_fft(fp12* v, size_t n, size_t count, bool inverse, bool sc
 
 // ---------------------------------------------------------------------------------------------
 int cyc_prefix_batched_enqueue(fp12* v, size_t n, size_t count, size_t out_count, uint32_t* tab,
-                                    hipStream_t s) {
-  if (cyc_tab_bytes(n, count) && !tab) return set_error(BTE_EINVAL, "cyclotomic prefix: missing odd-power storage");
-  return cyclotomic_fft(v, n, count, true, false, out_count, 1, s, tab);
+                               size_t c3max, hipStream_t s) {
+  if (cyc_tab_bytes(n, count, c3max) && !tab) return set_error(BTE_EINVAL, "cyclotomic prefix: missing odd-power storage");
+  return cyclotomic_fft(v, n, count, true, false, out_count, 1, s, tab, c3max);
 }
 
 // Builds the twiddle tables (and cyclotomic plans) of a domain ahead of launch-only use.
@@ -827,10 +829,11 @@ int cyc_prefix_ifft_segmented(fp12* v, const int* log_sizes, const uint32_t* out
   BTE_TRY_RC(sc.upload(&d_off, off.data(), count + 1));
   BTE_TRY_RC(sc.upload(&d_logs, logs.data(), count));
   BTE_TRY_RC(sc.upload(&d_order, order.data(), order.size()));
+  const size_t c3max = three_lane_max();
   size_t widest3 = 0;  // widest stage that takes the three-lane form
   for (int lh = 0; lh < logmax; ++lh) {
     const size_t nbf = start[lh + 1] - start[lh];
-    if (nbf < C3_MAX_BUTTERFLIES) widest3 = std::max(widest3, nbf);
+    if (nbf < c3max) widest3 = std::max(widest3, nbf);
   }
   uint32_t* tab = nullptr;
   if (widest3) BTE_TRY_RC(sc.dev((uint8_t**)&tab, (size_t)c3_blocks(widest3) * 64 * C3_TAB_BYTES));
@@ -838,7 +841,7 @@ int cyc_prefix_ifft_segmented(fp12* v, const int* log_sizes, const uint32_t* out
                      (const uint64_t*)d_off, (const uint8_t*)d_logs, (uint32_t)count, total);
   for (int lh = 0; lh < logmax; ++lh) {
     const uint64_t nbf = start[lh + 1] - start[lh];
-    if (nbf && nbf < C3_MAX_BUTTERFLIES)
+    if (nbf && nbf < c3max)
       hipLaunchKernelGGL(k_cyc_stage_ord3, dim3(c3_blocks(nbf)), dim3(64), 0, s, v,
                          (const uint32_t*)d_order + start[lh], nbf, (const uint64_t*)d_off, (const uint8_t*)d_logs,
                          (uint32_t)count, logmax, lh, (const cyc_plan*)t->plans, tab);

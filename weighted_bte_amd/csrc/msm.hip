@@ -46,7 +46,6 @@ namespace bte {
 // of the device so the last round is not left part-empty.  Kernels that need K recompute it from
 // the entry count E = offsets[nb].
 static constexpr uint32_t MSM_CHUNK_MIN = 16;
-static constexpr int ACC_ROUNDS = 2;
 BTE_DEV uint32_t chunk_size(uint32_t E, uint32_t tstar) {
   const uint32_t k = (uint32_t)(((uint64_t)E + tstar - 1) / tstar);
   return k > MSM_CHUNK_MIN ? k : MSM_CHUNK_MIN;
@@ -621,7 +620,7 @@ __global__ void __launch_bounds__(64) k_msm_combine(const jac<F>* __restrict__ w
 // ---------------------------------------------------------------------------------------------
 // Host driver
 // ---------------------------------------------------------------------------------------------
-// Accumulation threads: ACC_ROUNDS full-occupancy rounds of one-wavefront workgroups on this
+// Accumulation threads: BTE_TUNE_MSM_ACC_ROUNDS (default 2) full-occupancy rounds of one-wavefront workgroups on this
 // device (queried once per device), and no more than chunks of the minimum size need.
 template <class F>
 static uint32_t acc_threads(size_t E_max) {
@@ -643,7 +642,7 @@ static uint32_t acc_threads(size_t E_max) {
     }
     sl = slots[dev];
   }
-  const size_t full = (size_t)sl * ACC_BLOCK * ACC_ROUNDS;
+  const size_t full = (size_t)sl * ACC_BLOCK * (size_t)tune_get(BTE_TUNE_MSM_ACC_ROUNDS);
   const size_t need = (E_max + MSM_CHUNK_MIN - 1) / MSM_CHUNK_MIN;
   return (uint32_t)std::max<size_t>(1, std::min(full, need));
 }

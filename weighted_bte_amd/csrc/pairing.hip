@@ -373,14 +373,15 @@ extern "C" int bte_multi_miller_loop_noalloc(const bte_p1_affine* p, const bte_p
   return BTE_OK;
 }
 
-// Pairs sharing one accumulator: up to 8, fewer when that would leave fewer than 2^16 threads
-// (one wave per SIMD), since the shared squarings save less than idle SIMDs cost.
-static constexpr uint64_t MM_CHUNK_MAX = 8, MM_MIN_THREADS = 1 << 16;
+// Pairs sharing one accumulator: up to BTE_TUNE_MM_CHUNK_MAX (default 8), fewer when that would
+// leave fewer than BTE_TUNE_MM_MIN_THREADS threads (default 2^16, one wave per SIMD), since the
+// shared squarings save less than idle SIMDs cost.
 
 void bte::mm_chunks(const uint64_t* lo, const uint64_t* hi, size_t nseg, MMChunks& out) {
   uint64_t npairs = 0;
   for (size_t g = 0; g < nseg; ++g) npairs += hi[g] - lo[g];
-  const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(MM_CHUNK_MAX, npairs / MM_MIN_THREADS));
+  const uint64_t cmax = (uint64_t)tune_get(BTE_TUNE_MM_CHUNK_MAX), tmin = (uint64_t)tune_get(BTE_TUNE_MM_MIN_THREADS);
+  const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(cmax, npairs / tmin));
   out.lo.clear();
   out.hi.clear();
   out.seg.assign(1, 0);
