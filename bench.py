@@ -42,7 +42,7 @@ def measured_traffic(kernel):
         return None
     for line in open(TRAFFIC_SUMMARY):
         parts = [p.strip() for p in line.strip().strip("|").split("|")]
-        if len(parts) >= 5 and parts[0] == kernel:
+        if len(parts) >= 5 and (parts[0] == kernel or parts[0].startswith(kernel + "<")):
             try:
                 return int(parts[4].replace(",", ""))
             except ValueError:
