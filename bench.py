@@ -256,11 +256,14 @@ def bench_knob_tuning(bases, scalars):
     print("tuned three-lane switch: %d (default %d)" % (thr, d3), file=sys.stderr, flush=True)
     P, Q, off, _, _ = _multi_pairing_inputs()
     lines = api.prepare_g2_lines(Q)
-    dm = api.TUNE_DEFAULTS[api.TUNE_MM_CHUNK_MAX]
+    dm = "%d/%d" % (api.TUNE_DEFAULTS[api.TUNE_MM_CHUNK_MAX], api.TUNE_DEFAULTS[api.TUNE_MM_MIN_THREADS])
     best, times = autotune.tune_mm_chunk(lines, P, off)
-    out["mm_chunk_max"] = {"default": dm, "default_ms": times.get(dm), "tuned": best, "tuned_ms": times[best],
-                           "sweep_ms": times, "what": "pairs sharing one Miller accumulator, 512 uneven multi-pairings"}
-    print("tuned multi-Miller chunk: %d (default %d)" % (best, dm), file=sys.stderr, flush=True)
+    best = "%d/%d" % best
+    out["mm_chunk"] = {"default": dm, "default_ms": times[dm], "tuned": best, "tuned_ms": times[best],
+                       "sweep_ms": times,
+                       "what": "chunk max / min threads of the multi-Miller accumulators, 512 uneven multi-pairings"
+                               " (50,128 pairs: the built-in thread floor gives one pair per thread)"}
+    print("tuned multi-Miller chunk: %s (default %s)" % (best, dm), file=sys.stderr, flush=True)
     for knob, v in api.TUNE_DEFAULTS.items():
         api.tune_set(knob, v)
     return out
@@ -279,14 +282,15 @@ def bench_multi_pairing(nslots=512, seed=11):
     un, ms_un = _timed(lambda: api.full_final_exponentiation(api.unprepared_multi_miller_loop(P, Q, off)))
     lines, ms_lines = _timed(lambda: api.prepare_g2_lines(Q), reps=1)
     pr, ms_pr = _timed(lambda: api.full_final_exponentiation(api.multi_miller_loop_prepared(lines, P, off)))
+    fu, ms_fu = _timed(lambda: api.multi_pairing_prepared_fused(lines, P, off))
     check = [0, 1, nslots // 2, nslots - 1]
     exps = [sum(a * b for a, b in zip(a_s[off[s]:off[s + 1]], b_s[off[s]:off[s + 1]])) % R for s in check]
     want = api.to_host(api.cyclotomic_mul_scalar_decomposed(api.to_device(_gt_bytes() * len(check)), exps))
     got_un = b"".join(api.to_host(un[576 * s:576 * (s + 1)]) for s in check)
-    ok = got_un == want and torch.equal(un, pr)
+    ok = got_un == want and torch.equal(un, pr) and torch.equal(pr, fu)
     return {"slots": nslots, "pairs": npairs, "arity": "log-uniform 2..512",
             "unprepared_ms": round(ms_un, 3), "prepared_ms": round(ms_pr, 3),
-            "prepare_lines_ms": round(ms_lines, 3),
+            "prepare_lines_ms": round(ms_lines, 3), "prepared_one_launch_ms": round(ms_fu, 3),
             "pairs_per_s_unprepared": round(npairs / (ms_un * 1e-3), 1),
             "pairs_per_s_prepared": round(npairs / (ms_pr * 1e-3), 1), "parity": bool(ok)}
 

@@ -203,10 +203,22 @@ int bte_multi_miller_loop_noalloc(const bte_p1_affine* p, const bte_p2_affine* q
  * Identity pairs are skipped, empty = one. */
 int bte_multi_miller_loop_prepared(const bte_fp6* const* pair_tables, const bte_p1_affine* p, size_t npairs,
                                    const uint64_t* seg_offsets, size_t nseg, bte_fp12* out, void* stream);
+/* final_exponentiation(unprepared_multi_miller_loop(...)) per segment (final_exponentiation.rs:
+ * 229-273 then :348-354, the pairing product the reference's open_batch and share checks
+ * finalise), with the same arguments as bte_multi_miller_loop_prepared, in one kernel launch
+ * (after a small initialisation launch): the thread that finishes the last Miller chunk of a
+ * group of about sqrt(chunks) multiplies the group, the one that finishes a segment's last group
+ * multiplies the groups and runs the easy part, and its wavefront then runs the hard parts of the
+ * segments it closed on three lanes each.  out[s] equals bte_final_exp of
+ * bte_multi_miller_loop_prepared's out[s]; an empty segment gives one. */
+int bte_multi_pairing_prepared_fused(const bte_fp6* const* pair_tables, const bte_p1_affine* p, size_t npairs,
+                                     const uint64_t* seg_offsets, size_t nseg, bte_fp12* out, void* stream);
 /* final_exponentiation.rs:276-305 easy_final_exponentiation / batch_easy_final_exponentiation
  * (inputs nonzero; in and out must not alias) */
 int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
-/* final_exponentiation.rs:308-336 hard_final_exponentiation (in place allowed) */
+/* final_exponentiation.rs:308-336 hard_final_exponentiation (in place allowed); below the
+ * three-lane switch (BTE_TUNE_CYC_THREE_LANE_MAX elements) each element runs on three lanes,
+ * with the same values */
 int bte_final_exp_hard(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);
 /* final_exponentiation.rs:348-354 full_final_exponentiation (in and out must not alias) */
 int bte_final_exp(const bte_fp12* in, size_t n, bte_fp12* out, void* stream);

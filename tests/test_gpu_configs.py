@@ -135,7 +135,8 @@ def test_pfe_config4_fused_and_plain_vs_oracle(api, gt):
 def test_multi_pairing_512_uneven_slots_vs_oracle(api, gt):
     """512 multi-pairings of log-uniform arity 2..512 (about 50k pairs) in one call, unprepared
     and prepared, with one batched final exponentiation: slot s equals gT^(sum a_i b_i) over its
-    pairs (a_i G1, b_i G2) by the oracle on sampled slots, prepared == unprepared everywhere."""
+    pairs (a_i G1, b_i G2) by the oracle on sampled slots, prepared == unprepared == the
+    one-launch prepared multi-pairing everywhere."""
     import torch
 
     from weighted_bte_amd import inputs
@@ -157,9 +158,13 @@ def test_multi_pairing_512_uneven_slots_vs_oracle(api, gt):
             O.G1.to_affine(O.G1.mul(O.g1_gen_jac(), a_s[i]))
         assert O.g2_affine_from_bytes(api.to_host(Q[192 * i:192 * (i + 1)])) == \
             O.G2.to_affine(O.G2.mul(O.g2_gen_jac(), b_s[i]))
+    lines = api.prepare_g2_lines(Q)
     un = api.full_final_exponentiation(api.unprepared_multi_miller_loop(P, Q, off))
-    pr = api.full_final_exponentiation(api.multi_miller_loop_prepared(api.prepare_g2_lines(Q), P, off))
+    pr = api.full_final_exponentiation(api.multi_miller_loop_prepared(lines, P, off))
     assert torch.equal(un, pr)
+    # the same 512 pairings with the Miller loops, slot products and final exponentiations in one
+    # kernel launch
+    assert torch.equal(api.multi_pairing_prepared_fused(lines, P, off), pr)
     exps = [sum(a * b for a, b in zip(a_s[off[k]:off[k + 1]], b_s[off[k]:off[k + 1]])) for k in range(512)]
     check_sampled(api.to_host(un), exps, gt, random.Random(42))
 

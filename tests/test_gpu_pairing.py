@@ -95,6 +95,44 @@ def test_multi_miller_uneven_segments(api):
         assert got[s] == O.multi_pairing(pairs)
 
 
+def test_multi_pairing_fused_one_launch(api):
+    """The one-launch multi-pairing (Miller chunks, per-segment product and final exponentiation in
+    one kernel) equals the staged path bit for bit and the oracle's multi-pairing, over segments of
+    arity 3, 0, 1, 5, 2 and 12 with identity pairs (final_exponentiation.rs:229-273, :348-354),
+    followed by 30 segments of arity 1 or 2, so that one wavefront closes more segments than one
+    round of three-lane hard parts takes; with the chunk knob at 1 and at 16 pairs per
+    accumulator."""
+    import torch
+
+    rnd = random.Random(41)
+    sizes = [3, 0, 1, 5, 2, 12] + [1, 2, 1] * 10
+    n = sum(sizes)
+    ab = [(rnd.randrange(1, O.R), rnd.randrange(1, O.R)) for _ in range(n)]
+    P = [g1_mul_gen(a) for a, _ in ab]
+    Q = [g2_mul_gen(b) for _, b in ab]
+    P[1] = None  # identity inside segment 0
+    Q[7] = None  # identity inside segment 3
+    offs = [0]
+    for sz in sizes:
+        offs.append(offs[-1] + sz)
+    pd, qd = api.to_device(g1_points_bytes(P)), api.to_device(g2_points_bytes(Q))
+    lines = api.prepare_g2_lines(qd)
+    staged = api.full_final_exponentiation(api.multi_miller_loop_prepared(lines, pd, offs))
+    try:
+        for chunk in (1, 16):
+            api.tune_set(api.TUNE_MM_CHUNK_MAX, chunk)
+            api.tune_set(api.TUNE_MM_MIN_THREADS, 1)
+            fused = api.multi_pairing_prepared_fused(lines, pd, offs)
+            assert torch.equal(fused, staged), chunk
+    finally:
+        api.tune_set(api.TUNE_MM_CHUNK_MAX, api.TUNE_DEFAULTS[api.TUNE_MM_CHUNK_MAX])
+        api.tune_set(api.TUNE_MM_MIN_THREADS, api.TUNE_DEFAULTS[api.TUNE_MM_MIN_THREADS])
+    got = read_f12(api, fused, len(sizes))
+    for s_ in range(6):
+        pairs = [(P[i], Q[i]) for i in range(offs[s_], offs[s_ + 1])]
+        assert got[s_] == O.multi_pairing(pairs)
+
+
 def test_bilinearity_on_gpu(api):
     rnd = random.Random(5)
     a, b = rnd.randrange(1, O.R), rnd.randrange(1, O.R)

@@ -8,7 +8,7 @@ import random
 
 import pytest
 
-from helpers import O, g1_mul_gen, g2_mul_gen
+from helpers import O, g1_mul_gen, g1_points_bytes, g2_mul_gen, g2_points_bytes
 
 pytestmark = pytest.mark.gpu
 
@@ -83,3 +83,32 @@ def test_multi_miller_same_raw_values_for_every_chunk(api):
     for knob in (api.TUNE_MM_CHUNK_MAX, api.TUNE_MM_MIN_THREADS):
         api.tune_set(knob, api.TUNE_DEFAULTS[knob])
     assert all(torch.equal(outs[0], o) for o in outs[1:])
+
+
+def test_final_exponentiation_same_values_on_one_or_three_lanes(api):
+    """The hard part of the final exponentiation with one lane per element and with three lanes
+    per element (below the three-lane switch) gives the same values: 50 elements (workgroups of
+    21, the last one partial), through the hard part alone and through the full final
+    exponentiation (in place), equal to the oracle's final exponentiation on a few
+    (final_exponentiation.rs:308-354)."""
+    import torch
+
+    rnd = random.Random(23)
+    n = 50
+    P = [g1_mul_gen(rnd.randrange(1, O.R)) for _ in range(n)]
+    Q = [g2_mul_gen(rnd.randrange(1, O.R)) for _ in range(n)]
+    ml = api.miller_loop_pairs(api.to_device(g1_points_bytes(P)), api.to_device(g2_points_bytes(Q)))
+    easy = api.easy_final_exponentiation(ml)
+    hard, full = [], []
+    try:
+        for thr in (0, 1 << 30):  # every size on one lane, every size on three lanes
+            api.tune_set(api.TUNE_CYC_THREE_LANE_MAX, thr)
+            hard.append(api.hard_final_exponentiation(easy))
+            full.append(api.full_final_exponentiation(ml))
+    finally:
+        api.tune_set(api.TUNE_CYC_THREE_LANE_MAX, api.TUNE_DEFAULTS[api.TUNE_CYC_THREE_LANE_MAX])
+    assert torch.equal(hard[0], hard[1]) and torch.equal(full[0], full[1]) and torch.equal(hard[0], full[0])
+    raw = api.to_host(full[1])
+    for i in (0, 20, 21, 49):
+        want = O.final_exponentiation(O.miller_loop([(P[i], Q[i])]))
+        assert O.f12_from_bytes(raw[576 * i:576 * (i + 1)]) == want, i

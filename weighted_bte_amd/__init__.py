@@ -102,6 +102,7 @@ _SIGNATURES = {
     "bte_btx_verify_shares_batched": [_vp, _vp, _vp, _sz, _vp, _sz, _sz, _vp, _vp, _sz, _vp, _vp],
     "bte_pfe_open_prepared_batched": [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _sz, _vp, _vp],
     "bte_multi_miller_loop_prepared": [_vp, _vp, _sz, _vp, _sz, _vp, _vp],
+    "bte_multi_pairing_prepared_fused": [_vp, _vp, _sz, _vp, _sz, _vp, _vp],
     "bte_profile_enable": [_i],
     "bte_cyclotomic_prefix_work": [_sz, _sz, _vp],
     "bte_profile_read": [ctypes.c_char_p, _sz],

@@ -352,6 +352,21 @@ def multi_miller_loop_prepared(lines, p_affine, seg_offsets):
     return out
 
 
+def multi_pairing_prepared_fused(lines, p_affine, seg_offsets):
+    """Per segment, the final exponentiation of multi_miller_loop_prepared (the pairing product
+    final_exponentiation.rs:229-273 then :348-354), in one kernel launch."""
+    lines, p = _dev(lines), _dev(p_affine)
+    n = p.numel() // G1_AFFINE
+    if lines.numel() != n * LINES_BYTES or seg_offsets[0] != 0 or seg_offsets[-1] != n:
+        raise ValueError("lines / offsets do not match the points")
+    tabs = torch.arange(n, dtype=torch.int64, device=p.device) * LINES_BYTES + lines.data_ptr()
+    nseg = len(seg_offsets) - 1
+    offs = (ctypes.c_uint64 * len(seg_offsets))(*seg_offsets)
+    out = _empty(nseg * FP12, p)
+    call("bte_multi_pairing_prepared_fused", _ptr(tabs), _ptr(p), n, offs, nseg, _ptr(out), _stream())
+    return out
+
+
 def miller_loop_prepared(lines, p_affine):
     lines, p = _dev(lines), _dev(p_affine)
     n = p.numel() // G1_AFFINE

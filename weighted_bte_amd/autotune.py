@@ -11,8 +11,9 @@ blst_utils.rs:57-64; chunk and pair scheduling of the transforms, fft.rs:370-386
   * three-lane switch of the cyclotomic stages (BTE_TUNE_CYC_THREE_LANE_MAX): the stage width
     below which each Fp12 runs on three lanes instead of one.
         best, table = tune_three_lane(n=1024)
-  * multi-Miller chunk (BTE_TUNE_MM_CHUNK_MAX): pairs sharing one Miller accumulator.
-        best, times = tune_mm_chunk(lines, points, seg_offsets)
+  * multi-Miller chunk (BTE_TUNE_MM_CHUNK_MAX with BTE_TUNE_MM_MIN_THREADS): pairs sharing
+    one Miller accumulator.
+        (cmax, tmin), times = tune_mm_chunk(lines, points, seg_offsets)
 
     save_table({("g1", 20): best}, knobs={...})     # keyed by device name
     apply_table()                                   # re-applies saved choices in a new process
@@ -109,10 +110,30 @@ def tune_acc_rounds(points, scalars, group=1, candidates=(1, 2, 3, 4)):
     return tune_knob(api.TUNE_MSM_ACC_ROUNDS, candidates, lambda: fn(points, scalars))
 
 
-def tune_mm_chunk(lines, points, seg_offsets, candidates=(2, 4, 8, 16)):
-    """Pairs per multi-Miller accumulator on a prepared multi-pairing workload."""
-    return tune_knob(api.TUNE_MM_CHUNK_MAX, candidates,
-                     lambda: api.multi_miller_loop_prepared(lines, points, seg_offsets))
+def tune_mm_chunk(lines, points, seg_offsets, candidates=(1, 2, 4, 8, 16), reps=3):
+    """Pairs per multi-Miller accumulator on a prepared multi-pairing workload.  The chunk the
+    library uses is min(BTE_TUNE_MM_CHUNK_MAX, pairs / BTE_TUNE_MM_MIN_THREADS), so each
+    candidate c is timed with the thread floor at 1 (exactly c pairs per accumulator) beside the
+    built-in pair of values; installs the fastest pair and returns
+    ((chunk max, min threads), {"c/min": ms})."""
+    knobs = (api.TUNE_MM_CHUNK_MAX, api.TUNE_MM_MIN_THREADS)
+    old = tuple(api.tune_get(k) for k in knobs)
+    default = tuple(api.TUNE_DEFAULTS[k] for k in knobs)
+    settings = [default] + [(c, 1) for c in candidates]
+    times = {}
+    fn = lambda: api.multi_miller_loop_prepared(lines, points, seg_offsets)
+    try:
+        for pair in settings:
+            for k, v in zip(knobs, pair):
+                api.tune_set(k, v)
+            times[pair] = _time(fn, reps)
+    finally:
+        for k, v in zip(knobs, old):
+            api.tune_set(k, v)
+    best = min(times, key=times.get)
+    for k, v in zip(knobs, best):
+        api.tune_set(k, v)
+    return best, {"%d/%d" % pair: ms for pair, ms in times.items()}
 
 
 def tune_three_lane(n=1024, counts=(2, 8, 16, 32, 64, 128), reps=2):

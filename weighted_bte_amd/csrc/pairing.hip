@@ -28,6 +28,7 @@
 #include "common.hpp"
 #include "ctx.hpp"
 #include "curve.hpp"
+#include "coop12.hpp"
 #include "cyclo.hpp"
 
 namespace bte {
@@ -195,6 +196,188 @@ __global__ void __launch_bounds__(64) k_multi_miller_prepared(const line_t* cons
     }
   }
   out[j] = fp12_conj(f);
+}
+
+// ---- hard part on three lanes per element (coop12.hpp) -------------------------------------
+// The chain of hard_part (cyclo.hpp) with each Fp12 spread over three lanes: a third of the Fp
+// products per lane on the same values, which shortens the chain when too few elements fill the
+// GPU one per lane.  Inputs are cyclotomic (the easy part's outputs), as hard_part assumes.
+// Products and squares are out-of-line calls, as the one-lane Fp12 operations are, so the
+// kernels stay small; the exchange buffer is re-typed as LDS inside them so that the calls keep
+// LDS instructions (a plain pointer argument would turn them into flat accesses).
+typedef __attribute__((address_space(3))) uint32_t lds_word;
+
+BTE_DEV c3ctx c3_in_lds(const c3ctx& c0) {
+  c3ctx c = c0;
+  c.buf = (volatile uint32_t*)(volatile lds_word*)c0.buf;
+  return c;
+}
+BTE_NOINL fp4 c3_mul_out(const c3ctx& c, const fp4& A, const fp4& B) { return c3_mul(c3_in_lds(c), A, B); }
+BTE_NOINL fp4 c3_sqr_out(const c3ctx& c, const fp4& A) { return c3_cyclotomic_sqr(c3_in_lds(c), A); }
+
+// a^(|x|/2), conjugated: the runs of squarings of raise_to_z_div_by_2 (cyclo.hpp), one product
+// before each run.
+BTE_NOINL fp4 c3_raise_to_z_div_by_2(const c3ctx& c, const fp4& a) {
+  fp4 r = c3_sqr_out(c, a);
+#pragma unroll 1
+  for (int run = 0; run < 5; ++run) {
+    const int len = run == 0 ? 2 : run == 1 ? 3 : run == 2 ? 9 : run == 3 ? 32 : 15;
+    r = c3_mul_out(c, r, a);
+#pragma unroll 1
+    for (int i = 0; i < len; ++i) r = c3_sqr_out(c, r);
+  }
+  return c3_conj(c, r);
+}
+
+BTE_DEV fp4 c3_raise_to_z(const c3ctx& c, const fp4& a) { return c3_sqr_out(c, c3_raise_to_z_div_by_2(c, a)); }
+
+BTE_DEV fp4 c3_hard_part(const c3ctx& c, const fp4& f) {
+  const fp4 y0 = c3_sqr_out(c, f);
+  fp4 y1 = c3_raise_to_z(c, y0);
+  fp4 y2 = c3_raise_to_z_div_by_2(c, y1);
+  fp4 y3 = c3_conj(c, f);
+  y1 = c3_mul_out(c, y1, y3);
+  y1 = c3_conj(c, y1);
+  y1 = c3_mul_out(c, y1, y2);
+  y2 = c3_raise_to_z(c, y1);
+  y3 = c3_raise_to_z(c, y2);
+  y1 = c3_conj(c, y1);
+  y3 = c3_mul_out(c, y3, y1);
+  y1 = c3_conj(c, y1);
+  y1 = c3_frobenius(c, y1, 3);
+  y2 = c3_frobenius(c, y2, 2);
+  y1 = c3_mul_out(c, y1, y2);
+  y2 = c3_raise_to_z(c, y3);
+  y2 = c3_mul_out(c, y2, y0);
+  y2 = c3_mul_out(c, y2, f);
+  y1 = c3_mul_out(c, y1, y2);
+  y2 = c3_frobenius(c, y3, 1);
+  return c3_mul_out(c, y1, y2);
+}
+
+// 21 elements per 64-lane workgroup (lanes 3g, 3g+1, 3g+2 hold element g; lane 63 idles).
+static constexpr int C3_ELEMS_PER_WG = 21;
+
+// in == out is allowed (each lane group reads its element before it writes it).
+__global__ void __launch_bounds__(64) k_final_exp_hard3(const fp12* in, size_t n, fp12* out) {
+  __shared__ uint32_t xbuf[48 * 64];
+  const c3ctx c = c3_make(xbuf);
+  const size_t i = blockIdx.x * (size_t)C3_ELEMS_PER_WG + c.lane / 3;
+  if (c.lane >= 3 * C3_ELEMS_PER_WG || i >= n) return;
+  const fp4 f = c3_load(c, in + i);
+  c3_store(c, out + i, c3_hard_part(c, f));
+}
+
+// Multi-pairings in one launch: thread j runs chunk j of the prepared multi-Miller loop (as
+// k_multi_miller_prepared) and stores its value.  The chunks of a segment form groups of up to
+// `group` consecutive chunks; the thread that finishes a group's last chunk multiplies the
+// group's values and stores the group product, and the thread that stores a segment's last group
+// product multiplies those and applies the easy part of the final exponentiation; the wavefront
+// then runs the hard parts of the segments its lanes closed, three lanes per segment.  So no
+// launch boundary
+// separates the Miller loops from the final exponentiation, no thread waits for another, and the
+// longest serial chain is one group plus one segment's group products.  "Last" is decided by a
+// counter: each thread stores its value, fences at device scope (release), increments the
+// counter, and the one that sees the count complete fences again (acquire) before it reads the
+// others' values.  The values equal the staged path's (a product in Fp12 does not depend on the
+// order of its factors, and the easy and hard parts are the same maps).
+// seg_chunks: the chunks of segment s are seg_chunks[s] .. seg_chunks[s+1]; seg_groups: its
+// group products are gpart[seg_groups[s] .. seg_groups[s+1]]; the counters and out[] are set by
+// k_multi_pairing_init first (out[s] = 1 stays for a segment without pairs).
+__global__ void k_multi_pairing_init(uint32_t* __restrict__ seg_done, fp12* __restrict__ out, size_t nseg,
+                                     uint32_t* __restrict__ group_done, size_t ngroups) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < nseg) {
+    seg_done[i] = 0;
+    out[i] = fp12_one();
+  }
+  if (i < ngroups) group_done[i] = 0;
+}
+
+// Product of v[lo .. hi), hi > lo.
+BTE_DEV fp12 fp12_product(const fp12* v, uint64_t lo, uint64_t hi) {
+  fp12 acc = v[lo];
+#pragma unroll 1
+  for (uint64_t c = lo + 1; c < hi; ++c) acc = fp12_mul(acc, v[c]);
+  return acc;
+}
+
+__global__ void __launch_bounds__(64) k_multi_pairing_fused(const line_t* const* __restrict__ tables,
+                                                            const g1a* __restrict__ P,
+                                                            const uint64_t* __restrict__ chunk_lo,
+                                                            const uint64_t* __restrict__ chunk_hi,
+                                                            const uint64_t* __restrict__ seg_chunks,
+                                                            const uint64_t* __restrict__ seg_groups, uint32_t nseg,
+                                                            uint32_t group, size_t nchunks, fp12* part,
+                                                            fp12* gpart, uint32_t* seg_done, uint32_t* group_done,
+                                                            fp12* out) {
+  __shared__ uint32_t xbuf[48 * 64];
+  // Chunk j = lane * (workgroups) + workgroup: consecutive chunks (the chunks of a small segment)
+  // sit in different wavefronts, so the segments a wavefront closes, and their hard parts, spread
+  // over the grid instead of piling up in the wavefronts that hold many small segments.  Every
+  // lane of the wavefront stays to the end (lanes past the last chunk run an empty loop): the
+  // hard parts are finished by the whole wavefront together.
+  const size_t j = threadIdx.x * (size_t)gridDim.x + blockIdx.x;
+  const bool live = j < nchunks;
+  const uint64_t a = live ? chunk_lo[j] : 0, b = live ? chunk_hi[j] : 0;
+  fp12 f = fp12_one();
+  int k = 0;
+#pragma unroll 1
+  for (int bit = 62; bit >= 0; --bit) {
+    if (bit != 62) f = fp12_sqr(f);
+    const int nsteps = ((MILLER_X_ABS >> bit) & 1) ? 2 : 1;
+#pragma unroll 1
+    for (int st = 0; st < nsteps; ++st, ++k) {
+#pragma unroll 1
+      for (uint64_t g = a; g < b; ++g) {
+        const line_t* L = tables[g];
+        const g1a Pg = P[g];
+        if (aff_is_identity(Pg) || fp2_is_zero(L[0].cy)) continue;
+        f = apply_line(f, L[k], Pg);
+      }
+    }
+  }
+  bool fin = false;  // this lane closed segment s: out[s] holds its easy part
+  uint32_t s = 0;
+  if (live) {
+    part[j] = fp12_conj(f);
+    s = seg_of(seg_chunks, nseg, j);
+    const uint64_t c0 = seg_chunks[s], c1 = seg_chunks[s + 1];
+    const uint64_t gi = (j - c0) / group;
+    const uint64_t glo = c0 + gi * group, ghi = glo + group < c1 ? glo + group : c1;
+    const uint64_t gid = seg_groups[s] + gi;
+    __threadfence();
+    if (atomicAdd(&group_done[gid], 1u) + 1u == (uint32_t)(ghi - glo)) {
+      __threadfence();
+      gpart[gid] = fp12_product(part, glo, ghi);
+      const uint64_t g0 = seg_groups[s], g1 = seg_groups[s + 1];
+      __threadfence();
+      if (atomicAdd(&seg_done[s], 1u) + 1u == (uint32_t)(g1 - g0)) {
+        __threadfence();
+        const fp12 acc = fp12_product(gpart, g0, g1);
+        out[s] = easy_part_with_inverse(acc, fp12_inv(acc));
+        fin = true;
+      }
+    }
+  }
+  // Hard parts of the segments this wavefront closed, C3_ELEMS_PER_WG at a time on three lanes
+  // each: lane group q takes the q-th closing lane's segment.
+  __threadfence_block();
+  const uint64_t mask = __ballot(fin);
+  const int cnt = __popcll(mask);
+  const c3ctx c = c3_make(xbuf);
+#pragma unroll 1
+  for (int base = 0; base < cnt; base += C3_ELEMS_PER_WG) {
+    const int q = base + c.lane / 3;
+    uint64_t m = mask;
+#pragma unroll 1
+    for (int t = 0; t < q && m; ++t) m &= m - 1;
+    const uint32_t seg = __shfl(s, m ? __builtin_ctzll(m) : 0);
+    if (c.lane < 3 * C3_ELEMS_PER_WG && q < cnt) {
+      const fp4 e = c3_load(c, out + seg);
+      c3_store(c, out + seg, c3_hard_part(c, e));
+    }
+  }
 }
 
 // out[s] = prod of f[off[s] .. off[s+1]); one 64-thread workgroup per segment.
@@ -442,6 +625,54 @@ extern "C" int bte_multi_miller_loop_prepared(const bte_fp6* const* pair_tables,
                                (hipStream_t)stream);
 }
 
+extern "C" int bte_multi_pairing_prepared_fused(const bte_fp6* const* pair_tables, const bte_p1_affine* p,
+                                                size_t npairs, const uint64_t* seg_offsets, size_t nseg, bte_fp12* out,
+                                                void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (nseg == 0) return BTE_OK;
+  if (!seg_offsets || !out || (npairs && (!p || !pair_tables)))
+    return set_error(BTE_EINVAL, "multi_pairing_prepared_fused: null pointer");
+  if (seg_offsets[0] != 0 || seg_offsets[nseg] != npairs)
+    return set_error(BTE_EINVAL, "multi_pairing_prepared_fused: offsets must run from 0 to npairs");
+  for (size_t g = 0; g < nseg; ++g)
+    if (seg_offsets[g + 1] < seg_offsets[g])
+      return set_error(BTE_EINVAL, "multi_pairing_prepared_fused: bad segment offsets");
+  if (nseg > 0xffffffffu) return set_error(BTE_EINVAL, "multi_pairing_prepared_fused: too many segments");
+  MMChunks ch;
+  mm_chunks(seg_offsets, seg_offsets + 1, nseg, ch);
+  const size_t nchunks = ch.lo.size();
+  // Groups of about sqrt(c) chunks for the segment with the most chunks c: the serial products
+  // after the Miller loops are then about 2 sqrt(c) multiplications long.
+  uint64_t cmax = 1;
+  for (size_t g = 0; g < nseg; ++g) cmax = std::max<uint64_t>(cmax, ch.seg[g + 1] - ch.seg[g]);
+  uint64_t group = 1;
+  while (group * group < cmax) ++group;
+  std::vector<uint64_t> seg_groups(nseg + 1, 0);
+  for (size_t g = 0; g < nseg; ++g) seg_groups[g + 1] = seg_groups[g] + (ch.seg[g + 1] - ch.seg[g] + group - 1) / group;
+  const size_t ngroups = seg_groups[nseg];
+  CallScope sc(s);
+  uint64_t *d_lo, *d_hi, *d_seg, *d_groups;
+  fp12 *part, *gpart;
+  uint32_t *seg_done, *group_done;
+  BTE_TRY_RC(sc.upload(&d_lo, ch.lo.data(), nchunks));
+  BTE_TRY_RC(sc.upload(&d_hi, ch.hi.data(), nchunks));
+  BTE_TRY_RC(sc.upload(&d_seg, ch.seg.data(), ch.seg.size()));
+  BTE_TRY_RC(sc.upload(&d_groups, seg_groups.data(), seg_groups.size()));
+  BTE_TRY_RC(sc.dev(&part, nchunks));
+  BTE_TRY_RC(sc.dev(&gpart, ngroups));
+  BTE_TRY_RC(sc.dev(&seg_done, nseg));
+  BTE_TRY_RC(sc.dev(&group_done, ngroups));
+  hipLaunchKernelGGL(k_multi_pairing_init, dim3(blocks_for(std::max(nseg, ngroups), 64)), dim3(64), 0, s, seg_done,
+                     (fp12*)out, nseg, group_done, ngroups);
+  if (nchunks)
+    hipLaunchKernelGGL(k_multi_pairing_fused, dim3(blocks_for(nchunks, 64)), dim3(64), 0, s,
+                       (const line_t* const*)pair_tables, (const g1a*)p, (const uint64_t*)d_lo,
+                       (const uint64_t*)d_hi, (const uint64_t*)d_seg, (const uint64_t*)d_groups, (uint32_t)nseg,
+                       (uint32_t)group, nchunks, part, gpart, seg_done, group_done, (fp12*)out);
+  BTE_CHECK_LAUNCH();
+  return BTE_OK;
+}
+
 extern "C" int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, void* stream) {
   if (n == 0) return BTE_OK;
   if (!in || !out || (const void*)in == (const void*)out)
@@ -452,25 +683,34 @@ extern "C" int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, v
   return BTE_OK;
 }
 
+// Hard part of n elements: one lane per element, or three lanes per element when n is below the
+// three-lane switch (BTE_TUNE_CYC_THREE_LANE_MAX, the same switch as the cyclotomic stages), where
+// one lane per element leaves most SIMDs idle and the shorter per-lane chain wins.
+static void hard_part_enqueue(const fp12* in, size_t n, fp12* out, hipStream_t s) {
+  if (n < (size_t)tune_get(BTE_TUNE_CYC_THREE_LANE_MAX))
+    hipLaunchKernelGGL(k_final_exp_hard3, dim3(blocks_for(n, C3_ELEMS_PER_WG)), dim3(64), 0, s, in, n, out);
+  else
+    hipLaunchKernelGGL(k_final_exp_hard, dim3(blocks_for(n, 64)), dim3(64), 0, s, in, n, out);
+}
+
 extern "C" int bte_final_exp_hard(const bte_fp12* in, size_t n, bte_fp12* out, void* stream) {
   if (n == 0) return BTE_OK;
   if (!in || !out) return set_error(BTE_EINVAL, "final_exp_hard: null pointer");
-  hipLaunchKernelGGL(k_final_exp_hard, dim3(blocks_for(n, 64)), dim3(64), 0, (hipStream_t)stream,
-                     (const fp12*)in, n, (fp12*)out);
+  hard_part_enqueue((const fp12*)in, n, (fp12*)out, (hipStream_t)stream);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
 
 // The easy part shares an inversion over chunks of EASY_CHUNK elements (one thread per chunk);
-// the hard part then runs one thread per element, so every SIMD gets work at any n.
+// the hard part then runs one thread (or three lanes) per element, so every SIMD gets work at
+// any n.
 extern "C" int bte_final_exp(const bte_fp12* in, size_t n, bte_fp12* out, void* stream) {
   if (n == 0) return BTE_OK;
   if (!in || !out || (const void*)in == (const void*)out)
     return set_error(BTE_EINVAL, "final_exp: null or aliased buffers");
   hipLaunchKernelGGL(k_final_exp_easy, dim3(blocks_for((n + EASY_CHUNK - 1) / EASY_CHUNK, 64)), dim3(64), 0,
                      (hipStream_t)stream, (const fp12*)in, n, (fp12*)out);
-  hipLaunchKernelGGL(k_final_exp_hard, dim3(blocks_for(n, 64)), dim3(64), 0, (hipStream_t)stream, (const fp12*)out,
-                     n, (fp12*)out);
+  hard_part_enqueue((const fp12*)out, n, (fp12*)out, (hipStream_t)stream);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
