@@ -112,3 +112,25 @@ def test_final_exponentiation_same_values_on_one_or_three_lanes(api):
     for i in (0, 20, 21, 49):
         want = O.final_exponentiation(O.miller_loop([(P[i], Q[i])]))
         assert O.f12_from_bytes(raw[576 * i:576 * (i + 1)]) == want, i
+
+
+def test_easy_part_same_values_for_every_chunk_length(api):
+    """The easy part shares one Fp12 inversion over 1, 2 or 4 elements depending on n (one per
+    thread up to 65,536 elements): 131,077 elements (4 per inversion) equal the first 65,536
+    (1 per inversion) and the remaining 65,541 (2 per inversion) run alone, and two of them equal
+    the oracle's easy part (final_exponentiation.rs:276-305)."""
+    import torch
+
+    n1, n = 65536, 131077
+    g = torch.Generator(device="cpu").manual_seed(9)
+    v = torch.randint(0, 1 << 31, (n * 144,), dtype=torch.int64, generator=g).to(torch.int32).view(n, 12, 12)
+    v[:, :, 11] &= 0x0FFFFFFF  # every limb string below p
+    x = v.reshape(-1).view(torch.uint8).to("cuda")
+    whole = api.easy_final_exponentiation(x)
+    first = api.easy_final_exponentiation(x[:576 * n1].clone())
+    rest = api.easy_final_exponentiation(x[576 * n1:].clone())
+    assert torch.equal(whole, torch.cat([first, rest]))
+    raw_in, raw_out = api.to_host(x[:576 * 2]), api.to_host(whole[:576 * 2])
+    for i in range(2):
+        f = O.f12_from_bytes(raw_in[576 * i:576 * (i + 1)])
+        assert O.f12_from_bytes(raw_out[576 * i:576 * (i + 1)]) == O.easy_final_exponentiation(f)

@@ -277,9 +277,9 @@ __global__ void __launch_bounds__(64) k_final_exp_hard3(const fp12* in, size_t n
 // launch boundary
 // separates the Miller loops from the final exponentiation, no thread waits for another, and the
 // longest serial chain is one group plus one segment's group products.  "Last" is decided by a
-// counter: each thread stores its value, fences at device scope (release), increments the
-// counter, and the one that sees the count complete fences again (acquire) before it reads the
-// others' values.  The values equal the staged path's (a product in Fp12 does not depend on the
+// counter: each thread stores its value, fences at device scope (release only), increments the
+// counter, and the one that sees the count complete fences again (acquire only) before it reads
+// the others' values.  The values equal the staged path's (a product in Fp12 does not depend on the
 // order of its factors, and the easy and hard parts are the same maps).
 // seg_chunks: the chunks of segment s are seg_chunks[s] .. seg_chunks[s+1]; seg_groups: its
 // group products are gpart[seg_groups[s] .. seg_groups[s+1]]; the counters and out[] are set by
@@ -346,14 +346,14 @@ __global__ void __launch_bounds__(64) k_multi_pairing_fused(const line_t* const*
     const uint64_t gi = (j - c0) / group;
     const uint64_t glo = c0 + gi * group, ghi = glo + group < c1 ? glo + group : c1;
     const uint64_t gid = seg_groups[s] + gi;
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     if (atomicAdd(&group_done[gid], 1u) + 1u == (uint32_t)(ghi - glo)) {
-      __threadfence();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       gpart[gid] = fp12_product(part, glo, ghi);
       const uint64_t g0 = seg_groups[s], g1 = seg_groups[s + 1];
-      __threadfence();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       if (atomicAdd(&seg_done[s], 1u) + 1u == (uint32_t)(g1 - g0)) {
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const fp12 acc = fp12_product(gpart, g0, g1);
         out[s] = easy_part_with_inverse(acc, fp12_inv(acc));
         fin = true;
@@ -403,14 +403,14 @@ __global__ void __launch_bounds__(64) k_segment_product(const fp12* __restrict__
   if (tid == 0) out[s] = sh[0];
 }
 
-// Easy part over chunks of EASY_CHUNK elements sharing one Fp12 inversion (Montgomery's trick,
-// as batch_easy_final_exponentiation does for the whole batch; the values are identical).
-static constexpr int EASY_CHUNK = 4;
-__global__ void __launch_bounds__(64) k_final_exp_easy(const fp12* __restrict__ in, size_t n, fp12* __restrict__ out) {
+// Easy part over chunks of `chunk` elements sharing one Fp12 inversion (Montgomery's trick, as
+// batch_easy_final_exponentiation does for the whole batch; the values are identical).
+__global__ void __launch_bounds__(64) k_final_exp_easy(const fp12* __restrict__ in, size_t n, uint32_t chunk,
+                                                       fp12* __restrict__ out) {
   const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  const size_t start = t * EASY_CHUNK;
+  const size_t start = t * chunk;
   if (start >= n) return;
-  const size_t stop = start + EASY_CHUNK < n ? start + EASY_CHUNK : n;
+  const size_t stop = start + chunk < n ? start + chunk : n;
   // prefix products staged in out[]
   fp12 acc = fp12_one();
   for (size_t i = start; i < stop; ++i) {
@@ -673,12 +673,21 @@ extern "C" int bte_multi_pairing_prepared_fused(const bte_fp6* const* pair_table
   return BTE_OK;
 }
 
+// Easy part of n elements: one element per thread while that is at most one wavefront per SIMD
+// (EASY_FULL threads at this kernel's occupancy on 256 CUs), else 2 or 4 elements sharing an
+// inversion, which trades three Fp12 products per element for the inversion they share.
+static constexpr size_t EASY_FULL = 65536;
+static void easy_part_enqueue(const fp12* in, size_t n, fp12* out, hipStream_t s) {
+  const uint32_t chunk = n <= EASY_FULL ? 1 : n <= 2 * EASY_FULL ? 2 : 4;
+  hipLaunchKernelGGL(k_final_exp_easy, dim3(blocks_for((n + chunk - 1) / chunk, 64)), dim3(64), 0, s, in, n, chunk,
+                     out);
+}
+
 extern "C" int bte_final_exp_easy(const bte_fp12* in, size_t n, bte_fp12* out, void* stream) {
   if (n == 0) return BTE_OK;
   if (!in || !out || (const void*)in == (const void*)out)
     return set_error(BTE_EINVAL, "final_exp_easy: null or aliased buffers");
-  hipLaunchKernelGGL(k_final_exp_easy, dim3(blocks_for((n + EASY_CHUNK - 1) / EASY_CHUNK, 64)), dim3(64), 0,
-                     (hipStream_t)stream, (const fp12*)in, n, (fp12*)out);
+  easy_part_enqueue((const fp12*)in, n, (fp12*)out, (hipStream_t)stream);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
 }
@@ -701,15 +710,14 @@ extern "C" int bte_final_exp_hard(const bte_fp12* in, size_t n, bte_fp12* out, v
   return BTE_OK;
 }
 
-// The easy part shares an inversion over chunks of EASY_CHUNK elements (one thread per chunk);
+// The easy part shares an inversion over chunks of up to 4 elements (one thread per chunk);
 // the hard part then runs one thread (or three lanes) per element, so every SIMD gets work at
 // any n.
 extern "C" int bte_final_exp(const bte_fp12* in, size_t n, bte_fp12* out, void* stream) {
   if (n == 0) return BTE_OK;
   if (!in || !out || (const void*)in == (const void*)out)
     return set_error(BTE_EINVAL, "final_exp: null or aliased buffers");
-  hipLaunchKernelGGL(k_final_exp_easy, dim3(blocks_for((n + EASY_CHUNK - 1) / EASY_CHUNK, 64)), dim3(64), 0,
-                     (hipStream_t)stream, (const fp12*)in, n, (fp12*)out);
+  easy_part_enqueue((const fp12*)in, n, (fp12*)out, (hipStream_t)stream);
   hard_part_enqueue((const fp12*)out, n, (fp12*)out, (hipStream_t)stream);
   BTE_CHECK_LAUNCH();
   return BTE_OK;
