@@ -462,7 +462,7 @@ int bte_probe_mac_rate(double* macs_per_second, void* stream);
  * split; results never depend on these).  Set by the autotuner (weighted_bte_amd/autotune.py),
  * read by a driver when it sizes a launch; a plan keeps the values it was created with.
  *   BTE_TUNE_MSM_ACC_ROUNDS      full-occupancy rounds of bucket-accumulation threads, 1..16
- *                                (default 2): fewer rounds mean longer chunks per thread
+ *                                (default 3): fewer rounds mean longer chunks per thread
  *   BTE_TUNE_CYC_THREE_LANE_MAX  cyclotomic stages with fewer butterflies than this run each
  *                                Fp12 on three lanes, 0..2^30 (default 32768)
  *   BTE_TUNE_MM_CHUNK_MAX        pairs sharing one multi-Miller accumulator, 1..64 (default 8)

@@ -34,7 +34,7 @@ def test_msm_same_points_for_every_accumulation_round_count(api):
         api.tune_set(api.TUNE_MSM_ACC_ROUNDS, rounds)
         got = api.to_host(api.batch_normalize_g1(api.g1_multi_exp_affine_bytes(bases, dsc)))
         assert got == want, rounds
-    api.tune_set(api.TUNE_MSM_ACC_ROUNDS, 2)
+    api.tune_set(api.TUNE_MSM_ACC_ROUNDS, api.TUNE_DEFAULTS[api.TUNE_MSM_ACC_ROUNDS])
 
 
 def test_cyclotomic_prefix_same_raw_values_on_one_or_three_lanes(api):

@@ -97,8 +97,8 @@ def test_tune_knobs_round_trip_and_range_checks():
     for knob, default in api.TUNE_DEFAULTS.items():
         assert api.tune_get(knob) == default
     try:
-        api.tune_set(api.TUNE_MSM_ACC_ROUNDS, 3)
-        assert api.tune_get(api.TUNE_MSM_ACC_ROUNDS) == 3
+        api.tune_set(api.TUNE_MSM_ACC_ROUNDS, 4)
+        assert api.tune_get(api.TUNE_MSM_ACC_ROUNDS) == 4
         api.tune_set(api.TUNE_CYC_THREE_LANE_MAX, 0)
         assert api.tune_get(api.TUNE_CYC_THREE_LANE_MAX) == 0
         for knob, bad in ((api.TUNE_MSM_ACC_ROUNDS, 0), (api.TUNE_MSM_ACC_ROUNDS, 17), (api.TUNE_MM_CHUNK_MAX, 65),

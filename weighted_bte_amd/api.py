@@ -199,7 +199,7 @@ def set_msm_window(group, n, bits):
 
 # Launch-shape knobs of bte_tune_set (include/bte_hip.h); results never depend on them.
 TUNE_MSM_ACC_ROUNDS, TUNE_CYC_THREE_LANE_MAX, TUNE_MM_CHUNK_MAX, TUNE_MM_MIN_THREADS = 1, 2, 3, 4
-TUNE_DEFAULTS = {TUNE_MSM_ACC_ROUNDS: 2, TUNE_CYC_THREE_LANE_MAX: 32768, TUNE_MM_CHUNK_MAX: 8,
+TUNE_DEFAULTS = {TUNE_MSM_ACC_ROUNDS: 3, TUNE_CYC_THREE_LANE_MAX: 32768, TUNE_MM_CHUNK_MAX: 8,
                  TUNE_MM_MIN_THREADS: 65536}
 
 

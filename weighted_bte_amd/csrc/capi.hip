@@ -27,7 +27,7 @@ struct TuneKnob {
   int64_t lo, hi;
 };
 static TuneKnob g_tune[5] = {{{0}, 0, 0},
-                             {{2}, 1, 16},
+                             {{3}, 1, 16},
                              {{32768}, 0, (int64_t)1 << 30},
                              {{8}, 1, 64},
                              {{65536}, 1, (int64_t)1 << 24}};

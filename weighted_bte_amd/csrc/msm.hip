@@ -620,7 +620,7 @@ __global__ void __launch_bounds__(64) k_msm_combine(const jac<F>* __restrict__ w
 // ---------------------------------------------------------------------------------------------
 // Host driver
 // ---------------------------------------------------------------------------------------------
-// Accumulation threads: BTE_TUNE_MSM_ACC_ROUNDS (default 2) full-occupancy rounds of one-wavefront workgroups on this
+// Accumulation threads: BTE_TUNE_MSM_ACC_ROUNDS (default 3) full-occupancy rounds of one-wavefront workgroups on this
 // device (queried once per device), and no more than chunks of the minimum size need.
 template <class F>
 static uint32_t acc_threads(size_t E_max) {
