@@ -51,8 +51,9 @@ def measured_traffic(kernel):
 
 
 def msm_window_bits(n: int) -> int:
-    lg = n.bit_length() - 1
-    return max(4, min(16, lg - 4))
+    """The library's built-in window width for n points."""
+    from weighted_bte_amd import autotune
+    return autotune.default_window(n)
 
 
 def nonzero_signed_digits(scalar_bytes: bytes, c: int) -> int:

@@ -303,3 +303,20 @@ def test_autotune_window_keeps_result(api, tmp_path):
         assert O.g1_compress(g1_jac_affine(api.to_host(api.g1_multi_exp_affine_bytes(bases, dsc)))) == want
     finally:
         api.set_msm_window(1, n, 0)
+
+
+@pytest.mark.parametrize("logn", [14, 18, 19])
+def test_g1_msm_built_in_window_table(api, logn):
+    """The built-in window table at sizes where it picks 8, 13 and 16 bits (2^14, 2^18, 2^19;
+    the former log2 n - 4 rule gave 10, 14 and 15): closed form (sum k_i s_i) G with the bench's
+    10%-sparse scalars, compared as compressed bytes (blst_utils.rs:43-67)."""
+    from weighted_bte_amd import autotune, inputs
+
+    n = 1 << logn
+    assert autotune.default_window(n) == {14: 8, 18: 13, 19: 16}[logn]
+    st = inputs.Stream(77, logn)
+    ks, sc = st.scalars(n), inputs.msm_scalars(st, n)
+    bases = api.g1_generator_mul(api.to_device(inputs.pack_scalars(ks)))
+    out = api.g1_multi_exp_affine_bytes(bases, api.to_device(inputs.pack_scalars(sc)))
+    want = api.to_host(api.g1_generator_mul(api.to_device((sum(k * s for k, s in zip(ks, sc)) % O.R).to_bytes(32, "little"))))
+    assert api.to_host(api.batch_normalize_g1(out)) == want

@@ -3,7 +3,7 @@ rayon make the corresponding choices themselves in the reference (Pippenger wind
 blst_utils.rs:57-64; chunk and pair scheduling of the transforms, fft.rs:370-386).
 
   * Pippenger window width c per (group, log2 n): the plain per-call MSM picks
-    c = clamp(log2 n - 4, 4, 16).
+    c = log2 n - 4 up to 2^12, then 8 / 10 / 13 / 16 for log2 n up to 14 / 16 / 18 / above.
         best, times = tune_msm(1, points, scalars)
   * bucket-accumulation rounds (BTE_TUNE_MSM_ACC_ROUNDS): chunk length of the MSM's sorted
     entries per thread, as full-occupancy rounds of the device.
@@ -37,8 +37,11 @@ C_MIN, C_MAX = 4, 16
 
 
 def default_window(n: int) -> int:
+    """The library's built-in window width for n points (msm.hip default_window_bits)."""
     lg = max(0, n.bit_length() - 1)
-    return max(C_MIN, min(C_MAX, lg - 4))
+    if lg <= 12:
+        return max(C_MIN, lg - 4)
+    return 8 if lg <= 14 else 10 if lg <= 16 else 13 if lg <= 18 else 16
 
 
 def _msm_fn(group):

@@ -84,6 +84,18 @@ static int floor_log2(size_t n) {
   return lg;
 }
 
+// Built-in window width by log2 n.  Measured on MI355X (G1 and G2, 10% sparse scalars): widths
+// 8, 10, 13 and 16 run well and the widths between them badly (G1 2^16: 1.69 ms at 10 against
+// 2.03 at 12; 2^18: 2.59 at 13 against 3.86 at 14; 2^19: 3.84 at 16 against 11.3 at 15), so
+// the table keeps to those four from 2^13 up; below that, log2 n - 4.
+static int default_window_bits(int lg) {
+  if (lg <= 12) return std::max(4, lg - 4);
+  if (lg <= 14) return 8;
+  if (lg <= 16) return 10;
+  if (lg <= 18) return 13;
+  return 16;
+}
+
 static MsmShape msm_shape(int curve, size_t n) {
   int lg = floor_log2(n);
   int c;
@@ -91,7 +103,7 @@ static MsmShape msm_shape(int curve, size_t n) {
     std::lock_guard<std::mutex> lk(g_window_mu);
     c = g_window_override[curve][lg];
   }
-  if (c <= 0) c = std::max(4, std::min(16, lg - 4));
+  if (c <= 0) c = default_window_bits(lg);
   MsmShape s;
   s.c = c;
   s.nw = (256 + c - 1) / c;
