@@ -175,12 +175,12 @@ def bench_fft_sizes(with_cpu=True):
         ok = torch.equal(norm(inv(y)), norm(x))
         row = {"group": ("G1", "G2", "GT")[group - 1], "n": n, "forward_ms": round(ms, 3),
                "butterflies_per_s": round((n // 2) * logn / (ms * 1e-3), 1), "roundtrip_ok": bool(ok)}
-        if group != 3 and logn <= 8:
+        if group != 3 and logn <= 10:
             # small transforms run as one batched MSM of the n x n DFT; the staged radix-2 path
             # for comparison (same outputs)
             W.call("bte_fft_set_dft_max_log", 0)
             y2, ms2 = _timed(lambda: fwd(x))
-            W.call("bte_fft_set_dft_max_log", 8)
+            W.call("bte_fft_set_dft_max_log", 10)
             row.update({"path": "dft_as_msm", "staged_ms": round(ms2, 3),
                         "paths_agree": bool(torch.equal(norm(y2), norm(y)))})
         if lib is not None and group != 2 and logn <= 12:

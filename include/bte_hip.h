@@ -118,10 +118,10 @@ int bte_g2_decompress(const uint8_t* in, size_t n, bte_p2_affine* points, uint8_
 int bte_fft_fr(uint8_t* values, size_t n, int inverse, int scaled, void* stream);
 int bte_fft_fr_batched(uint8_t* values, size_t n, size_t count, int inverse, int scaled, void* stream);
 
-/* G1/G2 transforms of size 2^2 .. 2^log_n (default 8) with n^2 x count <= 2^20 run as one
+/* G1/G2 transforms of size 2^2 .. 2^log_n (default 10) with n^2 x count <= 2^20 run as one
  * batched MSM per call (X_k = sum_j w^(jk) x_j, one segment per output) instead of log n
  * latency-bound stages; 0 turns that off.  The outputs are the same points.  No counterpart in
- * the reference (its transform is always staged); set by the autotuner. */
+ * the reference (its transform is always staged). */
 int bte_fft_set_dft_max_log(int log_n);
 
 /* weighted_btx/src/fft.rs:197-234 Radix2Domain::{fft, ifft, ifft_unscaled} over G1Projective */

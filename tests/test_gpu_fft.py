@@ -293,9 +293,10 @@ def _jac_from_affine_device(api, aff, group):
 
 @pytest.mark.parametrize("group", [1, 2])
 def test_small_transforms_by_msm_equal_staged_and_closed_form(api, group):
-    """Transforms up to 2^8 run as one batched MSM (X_k = sum_j w^(jk) x_j): the same points as
-    the staged radix-2 transform for forward, inverse and scaled inverse, single and batched,
-    with an identity input; and the closed form DFT(s) * G at 2^8."""
+    """Transforms up to 2^10 (n^2 x count <= 2^20) run as one batched MSM (X_k = sum_j w^(jk)
+    x_j): the same points as the staged radix-2 transform for forward, inverse and scaled inverse,
+    single and batched up to the 2^10 cutoff, with an identity input; and the closed form
+    DFT(s) * G at 2^8."""
     import ctypes
 
     import torch
@@ -308,19 +309,19 @@ def test_small_transforms_by_msm_equal_staged_and_closed_form(api, group):
     fn = "bte_fft_g1_batched" if group == 1 else "bte_fft_g2_batched"
     width = 144 if group == 1 else 288
     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    for logn, count in ((2, 1), (6, 3), (8, 1)):
+    for logn, count in ((2, 1), (6, 3), (8, 1), (9, 2), (10, 1)):
         n = 1 << logn
         s = [rnd.randrange(O.R) for _ in range(n * count)]
         s[1] = 0  # an identity input
         x = _jac_from_affine_device(api, gen(api.to_device(b"".join(v.to_bytes(32, "little") for v in s))), group)
         for inverse, scaled in ((0, 0), (1, 1), (1, 0)):
             outs = []
-            for max_log in (8, 0):
+            for max_log in (10, 0):
                 call("bte_fft_set_dft_max_log", max_log)
                 y = x.clone()
                 call(fn, ctypes.c_void_p(y.data_ptr()), n, count, inverse, scaled, st)
                 outs.append(norm(y))
-            call("bte_fft_set_dft_max_log", 8)
+            call("bte_fft_set_dft_max_log", 10)
             assert torch.equal(outs[0], outs[1]), (logn, count, inverse, scaled)
             assert outs[0].numel() == n * count * width // 3 * 2
         if logn == 8:

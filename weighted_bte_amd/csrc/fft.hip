@@ -440,7 +440,7 @@ struct DftScalars {
 };
 static std::mutex g_dft_mu;
 static std::map<std::tuple<int, int, int>, DftScalars> g_dft;  // (device, logn, 0 fwd / 1 inv / 2 inv scaled)
-static std::atomic<int> g_dft_max_log{8};                      // sizes up to 2^8 (bte_fft_set_dft_max_log)
+static std::atomic<int> g_dft_max_log{10};                     // sizes up to 2^10 (bte_fft_set_dft_max_log)
 static constexpr size_t DFT_MAX_TERMS = (size_t)1 << 20;
 
 static int dft_scalars(int logn, int kind, const uint8_t** out) {
