@@ -240,8 +240,9 @@ def bench_knob_tuning(bases, scalars):
     """Autotunes the launch-shape knobs (bte_tune_set) on their own workloads and reports
     default against tuned: the bucket-accumulation rounds on the headline MSM, the three-lane
     switch of the cyclotomic stages (batched prefix inverse transforms of size 1024, 2..128 at
-    a time) and the pairs per multi-Miller accumulator (512 prepared multi-pairings of arity
-    2..512).  Runs after every other measurement and leaves the built-in values installed."""
+    a time), the pairs per multi-Miller accumulator (512 prepared multi-pairings of arity
+    2..512) and the DFT cutoff (a G2 transform of 2^10 as one MSM or staged).  Runs after every
+    other measurement and leaves the built-in values installed."""
     from weighted_bte_amd import api, autotune
 
     out = {}
@@ -265,8 +266,16 @@ def bench_knob_tuning(bases, scalars):
                        "what": "chunk max / min threads of the multi-Miller accumulators, 512 uneven multi-pairings"
                                " (50,128 pairs: the built-in thread floor gives one pair per thread)"}
     print("tuned multi-Miller chunk: %s (default %s)" % (best, dm), file=sys.stderr, flush=True)
+    from weighted_bte_amd import inputs
+    st = inputs.Stream(12, 10)
+    x = _affine_to_jac(api, api.g2_generator_mul(api.to_device(inputs.pack_scalars(st.scalars(1 << 10)))), 2)
+    cut, times = autotune.tune_dft_cutoff(2, x)
+    out["dft_max_log"] = {"default": autotune.DFT_MAX_LOG_DEFAULT, "tuned": cut, "sweep_ms": times,
+                          "what": "largest transform run as one batched MSM, timed on a G2 transform of 2^10"}
+    print("tuned DFT cutoff: 2^%d (default 2^%d)" % (cut, autotune.DFT_MAX_LOG_DEFAULT), file=sys.stderr, flush=True)
     for knob, v in api.TUNE_DEFAULTS.items():
         api.tune_set(knob, v)
+    autotune.set_dft_max_log(autotune.DFT_MAX_LOG_DEFAULT)
     return out
 
 

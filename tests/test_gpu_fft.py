@@ -334,3 +334,29 @@ def test_small_transforms_by_msm_equal_staged_and_closed_form(api, group):
                 pt = (O.g1_affine_from_bytes if group == 1 else O.g2_affine_from_bytes)(got[aw * k:aw * (k + 1)])
                 mul = g1_mul_gen if group == 1 else g2_mul_gen
                 assert pt == (mul(want[k]) if want[k] else None), "closed form k=%d" % k
+
+
+def test_dft_cutoff_tuner_keeps_result(api, tmp_path):
+    """The DFT-cutoff tuner times a G1 transform of 2^9 points in the MSM form and staged,
+    installs the faster cutoff (9 or 8), and the transform's points do not change; the cutoff
+    round-trips through the autotuner table."""
+    import torch
+
+    from weighted_bte_amd import autotune
+
+    rnd = random.Random(91)
+    n = 1 << 9
+    s = [rnd.randrange(O.R) for _ in range(n)]
+    x = _jac_from_affine_device(api, api.g1_generator_mul(api.to_device(b"".join(v.to_bytes(32, "little") for v in s))), 1)
+    d = api.Radix2Domain(n)
+    before = api.batch_normalize_g1(d.fft_g1(x))
+    try:
+        cut, times = autotune.tune_dft_cutoff(1, x, reps=1)
+        assert cut in (8, 9) and set(times) == {"msm_ms", "staged_ms"}
+        assert torch.equal(api.batch_normalize_g1(d.fft_g1(x)), before)
+        path = str(tmp_path / "t.json")
+        autotune.save_table({}, path, knobs={"dft_max_log": 8})
+        assert autotune.apply_table(path) == 1
+        assert torch.equal(api.batch_normalize_g1(d.fft_g1(x)), before)
+    finally:
+        autotune.set_dft_max_log(autotune.DFT_MAX_LOG_DEFAULT)

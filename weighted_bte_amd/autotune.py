@@ -14,6 +14,9 @@ blst_utils.rs:57-64; chunk and pair scheduling of the transforms, fft.rs:370-386
   * multi-Miller chunk (BTE_TUNE_MM_CHUNK_MAX with BTE_TUNE_MM_MIN_THREADS): pairs sharing
     one Miller accumulator.
         (cmax, tmin), times = tune_mm_chunk(lines, points, seg_offsets)
+  * largest G1/G2 transform run as one batched MSM instead of staged (bte_fft_set_dft_max_log,
+    built-in 2^10).
+        cutoff, times = tune_dft_cutoff(2, jacobian_points_of_size_2_to_the_10)
 
     save_table({("g1", 20): best}, knobs={...})     # keyed by device name
     apply_table()                                   # re-applies saved choices in a new process
@@ -177,6 +180,41 @@ def tune_three_lane(n=1024, counts=(2, 8, 16, 32, 64, 128), reps=2):
     return thr, rows
 
 
+DFT_MAX_LOG_DEFAULT = 10
+
+
+def set_dft_max_log(log_n):
+    """G1/G2 transforms of size up to 2^log_n (with n^2 x count <= 2^20) run as one batched MSM
+    (bte_fft_set_dft_max_log; 0 = always staged)."""
+    from . import call
+    call("bte_fft_set_dft_max_log", int(log_n))
+
+
+def tune_dft_cutoff(group, values, reps=3):
+    """Times the forward transform of `values` (n Jacobian G1 or G2 points, n = 2^logn with
+    2 <= logn <= 10) in the MSM form and in the staged form and installs the cutoff that takes the
+    faster one: logn when the MSM form wins, logn - 1 otherwise (no result depends on it).
+    Returns (cutoff, {"msm_ms": ..., "staged_ms": ...})."""
+    from .api import G1_JAC, G2_JAC, Radix2Domain
+
+    n = values.numel() // (G1_JAC if group == 1 else G2_JAC)
+    logn = n.bit_length() - 1
+    if n != 1 << logn or not 2 <= logn <= 10:
+        raise ValueError("tune_dft_cutoff: size must be 2^2 .. 2^10")
+    d = Radix2Domain(n)
+    fwd = d.fft_g1 if group == 1 else d.fft_g2
+    times = {}
+    try:
+        for name, cut in (("msm_ms", logn), ("staged_ms", logn - 1)):
+            set_dft_max_log(cut)
+            times[name] = _time(lambda: fwd(values), reps)
+    finally:
+        set_dft_max_log(DFT_MAX_LOG_DEFAULT)
+    best = logn if times["msm_ms"] <= times["staged_ms"] else logn - 1
+    set_dft_max_log(best)
+    return best, times
+
+
 KNOBS = {"msm_acc_rounds": 1, "cyc_three_lane_max": 2, "mm_chunk_max": 3, "mm_min_threads": 4}
 
 
@@ -185,8 +223,8 @@ def _device_key():
 
 
 def save_table(choices, path=TABLE, knobs=None):
-    """choices: {("g1" | "g2", log2 n): c}; knobs: {name in KNOBS: value}; merged into the
-    table of this device."""
+    """choices: {("g1" | "g2", log2 n): c}; knobs: {name in KNOBS or "dft_max_log": value};
+    merged into the table of this device."""
     data = {}
     if os.path.exists(path):
         with open(path) as f:
@@ -195,7 +233,7 @@ def save_table(choices, path=TABLE, knobs=None):
     for (group, logn), c in choices.items():
         dev["%s_%d" % (group, logn)] = int(c)
     for name, v in (knobs or {}).items():
-        if name not in KNOBS:
+        if name not in KNOBS and name != "dft_max_log":
             raise ValueError("unknown knob %r" % name)
         dev[name] = int(v)
     os.makedirs(os.path.dirname(path), exist_ok=True)
@@ -213,6 +251,8 @@ def apply_table(path=TABLE):
     for key, c in data.get(_device_key(), {}).items():
         if key in KNOBS:
             api.tune_set(KNOBS[key], int(c))
+        elif key == "dft_max_log":
+            set_dft_max_log(int(c))
         else:
             group, logn = key.split("_")
             api.set_msm_window(1 if group == "g1" else 2, 1 << int(logn), int(c))
